@@ -1,0 +1,168 @@
+/*
+ * qrochet_hip.h -- C ABI of libqrochet_hip.so, the MI355X (gfx950) implementation
+ * of Qrochet.jl's MPS (`Chain` ansatz) update path.
+ *
+ * Every entry point replaces one method of the reference's Julia API
+ * (Qrochet.jl v0.1.2); the citation above each declaration names it.  There is
+ * no FFI in the reference -- its boundary is Julia method dispatch -- so a
+ * Julia shim `ccall`s these functions (a Python ctypes mirror ships beside the
+ * library).  Conventions:
+ *
+ *  - Sites and bonds are 1-based like the reference: bond b joins sites b and b+1.
+ *  - Complex numbers are interleaved (re, im) doubles, i.e. Julia ComplexF64 /
+ *    C `double _Complex` / numpy complex128.
+ *  - Host site arrays use the reference's default index order (o, l, r),
+ *    column-major (Julia memory order): site 1 is (o, r), site n is (o, l)
+ *    (`src/Ansatz/Chain.jl:33,64-100`).
+ *  - Gates are column-major (out_1.., in_1..) with lanes ascending, i.e.
+ *    G[out_l, out_r, in_l, in_r] for a 2-site gate (`src/Ansatz/Dense.jl:21-34`).
+ *  - The library copies host data in and out; it never keeps host pointers.
+ *  - Every call returns a qh_status; qh_last_error() gives the message.
+ *  - One context owns one HIP stream; calls on one context are serialised
+ *    in stream order.  Distinct contexts may be used from distinct threads.
+ */
+#ifndef QROCHET_HIP_H
+#define QROCHET_HIP_H
+
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qh_context qh_context;
+typedef struct qh_mps qh_mps;
+
+typedef enum {
+    QH_OK = 0,
+    QH_ERR_ARGUMENT = -1,        /* Julia ArgumentError */
+    QH_ERR_MISSING_SCHMIDT = -2, /* MissingSchmidtCoefficientsException (src/Ansatz.jl:91-99) */
+    QH_ERR_DIMENSION = -3,       /* DimensionMismatch / shape errors */
+    QH_ERR_ASSERTION = -4,       /* AssertionError (sites / ndims checks) */
+    QH_ERR_OOM = -5,             /* device allocation failed */
+    QH_ERR_HIP = -6,             /* HIP runtime error */
+    QH_ERR_NOT_CONVERGED = -7,   /* Jacobi SVD hit its sweep limit */
+    QH_ERR_ADMISSION = -8,       /* serving: request footprint above the budget */
+    QH_ERR_NUMERIC = -9          /* NaN/Inf detected in a decomposition */
+} qh_status;
+
+/* evolve flags */
+#define QH_CANONICAL 1u    /* iscanonical=true  (Vidal form, Chain.jl:669-722) */
+#define QH_RENORMALIZE 2u  /* renormalize=true  (Chain.jl:651-657) */
+/* contraction precision (the SVD and truncation decision stay FP64) */
+#define QH_PREC_FP64 0u
+#define QH_PREC_FP32 16u   /* θ build on FP32 MFMA, FP32 accumulation */
+#define QH_PREC_BF16X3 32u /* θ build as split-bf16 (3 products) MFMA, FP32 accumulation */
+#define QH_PREC_MASK 48u
+
+/* directions / methods of canonize_site! (Chain.jl:339-376) */
+#define QH_LEFT 0
+#define QH_RIGHT 1
+#define QH_METHOD_QR 0
+#define QH_METHOD_SVD 1
+
+/* "nothing" for threshold (NaN) / maxdim (negative) */
+#define QH_NO_THRESHOLD NAN
+#define QH_NO_MAXDIM (-1)
+
+/* ---- context ---------------------------------------------------------- */
+/* No reference counterpart (the reference has no device state). */
+int qh_ctx_create(int device, qh_context** out);
+int qh_ctx_destroy(qh_context* ctx);
+int qh_ctx_synchronize(qh_context* ctx);
+const char* qh_last_error(const qh_context* ctx);
+/* Kernel-class timing with HIP events on the context stream (0 = off). */
+int qh_ctx_set_timing(qh_context* ctx, int enable);
+/* Accumulated timing since the last reset: total ms and launch count of the
+ * named kernel class ("jacobi_round", "theta", "gemm", "writeback", ...). */
+int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_ms, int64_t* launches,
+                      double* algorithmic_flops, double* algorithmic_bytes);
+int qh_ctx_reset_timing(qh_context* ctx);
+/* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default. */
+int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);
+
+/* ---- MPS objects -------------------------------------------------------- */
+/* Replaces `MPS(arrays)` / `Chain(::State, ::Open, arrays)` (src/Ansatz/Chain.jl:18,64-100)
+ * followed by `adapt` to the device (ext/QrochetAdaptExt.jl:9).
+ * bond_dims: n-1 entries.  arrays[i]: (o,l,r) column-major, boundary 2-D.
+ * lambdas: NULL, or n-1 pointers (NULL where the bond has no Λ vector). */
+int qh_mps_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays,
+                  const double* const* lambdas, qh_mps** out);
+/* `copy`/`deepcopy` (src/Ansatz/Chain.jl:11): independent device copy. */
+int qh_mps_clone(qh_context* ctx, const qh_mps* src, qh_mps** out);
+int qh_mps_free(qh_mps* mps);
+int qh_mps_nsites(const qh_mps* mps, int* n, int* d);
+/* Bond dimensions (n-1) and Λ presence flags (n-1, may be NULL). */
+int qh_mps_bond_dims(qh_mps* mps, int* bond_dims, int* has_lambda);
+/* Download of one site tensor (o,l,r) column-major -- the `Chain(::HIPChain)` direction. */
+int qh_mps_get_site(qh_mps* mps, int site, double* host_array);
+/* Λ vector of a bond; returns QH_ERR_MISSING_SCHMIDT if the bond has none
+ * (`tensors(ψ; between)`, src/Ansatz.jl:78-89). */
+int qh_mps_get_lambda(qh_mps* mps, int bond, double* host_lambda);
+/* Device footprint in bytes (site tensors + Λ, current capacity). */
+int qh_mps_footprint(const qh_mps* mps, size_t* bytes);
+
+/* ---- gate application --------------------------------------------------- */
+/* `evolve_1site!` (src/Ansatz/Chain.jl:586-603): gate is d*d, (out, in) column-major. */
+int qh_evolve1(qh_context* ctx, qh_mps* mps, int site, const double* gate);
+/* `evolve!(ψ, gate; threshold, maxdim, iscanonical, renormalize)` for a 2-site gate
+ * (src/Ansatz/Chain.jl:543-584 -> evolve_2site! :606-661, contract_2sitewf! :669-685,
+ * unpack_2sitewf! :693-722, truncate! :390-422).  gate: d^4 complex, (out_l,out_r,in_l,in_r).
+ * threshold NaN -> nothing; maxdim < 0 -> nothing.  kept / discarded may be NULL; if
+ * given, the call waits for the result: kept = new bond dimension, discarded =
+ * sqrt(sum of squared dropped singular values). */
+int qh_evolve2(qh_context* ctx, qh_mps* mps, int bond, const double* gate, double threshold, int maxdim,
+               unsigned flags, int* kept, double* discarded);
+/* A brickwork layer: `count` independent 2-site updates, each on its own MPS or on
+ * disjoint bonds of a shared one, in one pipeline of batched launches (no
+ * reference counterpart as one call: it is `count` calls of evolve! fused).
+ * gates: count*d^4 complex.  kept/discarded: count entries or NULL. */
+int qh_evolve2_batch(qh_context* ctx, int count, qh_mps* const* mps, const int* bonds, const double* gates,
+                     double threshold, int maxdim, unsigned flags, int* kept, double* discarded);
+
+/* ---- truncation & canonical forms --------------------------------------- */
+/* `truncate!(ψ, bond; threshold, maxdim)` (src/Ansatz/Chain.jl:390-422). */
+int qh_truncate(qh_context* ctx, qh_mps* mps, int bond, double threshold, int maxdim);
+/* `contract!(ψ, :between, site1, site2; direction, delete_Λ)` (src/Ansatz/Chain.jl:309-333).
+ * direction QH_LEFT: into site1, QH_RIGHT: into site2. */
+int qh_contract_between(qh_context* ctx, qh_mps* mps, int site1, int site2, int direction, int delete_lambda);
+/* `canonize_site!(ψ, site; direction, method)` (src/Ansatz/Chain.jl:339-376). */
+int qh_canonize_site(qh_context* ctx, qh_mps* mps, int site, int direction, int method);
+/* `canonize!(ψ)` (src/Ansatz/Chain.jl:469-497): Vidal form. */
+int qh_canonize(qh_context* ctx, qh_mps* mps);
+/* `mixed_canonize!(ψ, center)` (src/Ansatz/Chain.jl:509-524). */
+int qh_mixed_canonize(qh_context* ctx, qh_mps* mps, int center);
+/* `normalize!(ψ, root)` (src/Ansatz/Chain.jl:532-536). */
+int qh_normalize(qh_context* ctx, qh_mps* mps, int root);
+/* `isleftcanonical` / `isrightcanonical` (src/Ansatz/Chain.jl:424-458):
+ * result = 1 if ||A^H A - I||_F <= atol.  dev (may be NULL) receives ||A^H A - I||_F. */
+int qh_is_left_canonical(qh_context* ctx, qh_mps* mps, int site, double atol, int* result, double* dev);
+int qh_is_right_canonical(qh_context* ctx, qh_mps* mps, int site, double atol, int* result, double* dev);
+
+/* ---- contractions --------------------------------------------------------- */
+/* One observable: nsites = 1 (op d*d) or 2 (op d^4, on sites (site, site+1)),
+ * column-major (out.., in..) like a gate. */
+typedef struct {
+    int nsites;
+    int site;
+    const double* op;
+} qh_observable;
+/* `expect(ψ, observables)` (src/Ansatz/Chain.jl:724-735): ⟨ψ|O_k…O_1|ψ⟩, unnormalised,
+ * one complex scalar for the whole list (applied in list order). */
+int qh_expect(qh_context* ctx, qh_mps* mps, int nobs, const qh_observable* obs, double* out);
+/* Many independent expectations from one pair of environment sweeps: term t is
+ * the product of obs[offsets[t] .. offsets[t+1]) (no reference counterpart as one
+ * call: it is nterms calls of expect).  out: nterms complex. */
+int qh_expect_many(qh_context* ctx, qh_mps* mps, int nterms, const int* offsets, const qh_observable* obs,
+                   double* out);
+/* `overlap(a, b)` (src/Ansatz/Chain.jl:737-748): Σ a·conj(b) = ⟨b|a⟩. */
+int qh_overlap(qh_context* ctx, qh_mps* a, qh_mps* b, double* out);
+/* `norm(ψ)` (src/Ansatz.jl:101-109). */
+int qh_norm(qh_context* ctx, qh_mps* mps, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

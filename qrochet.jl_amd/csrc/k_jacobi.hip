@@ -1,0 +1,420 @@
+// Batched block one-sided (Hestenes) Jacobi SVD in complex FP64, and the
+// finalisation kernels: singular values, sort, the `truncate!` rule
+// (src/Ansatz/Chain.jl:404-417) and the write-back of the kept factors.
+//
+// One launch = one round of a round-robin (circle-method) tournament over the
+// column blocks of every matrix in the batch.  A workgroup owns one block pair
+// (I, J) of one matrix: it forms the Gram matrix G = X_IJ^H X_IJ (BC × BC) from
+// HBM, and if any off-diagonal |g_ij| exceeds tol·sqrt(g_ii g_jj) it
+// diagonalises G with a cyclic two-sided Jacobi in LDS (FP64) and applies the
+// accumulated unitary Q to the columns of X and V:  X_IJ <- X_IJ Q,
+// V_IJ <- V_IJ Q.  Convergence is judged on freshly computed inner products, so
+// the singular values carry the accuracy of one-sided Jacobi (relative
+// accuracy on each σ_j = ||X e_j||).
+#include "qh_kernels.h"
+#include "qh_complex.h"
+
+namespace qh {
+
+__device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int& b) {
+    const int m = nidx - 1;
+    if (k == 0) { a = round; b = m; }
+    else { a = (round + k) % m; b = (round - k + m) % m; }
+}
+
+template <int BC>
+__global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
+                                                      const int2* __restrict__ tiles, int round, double tol) {
+    constexpr int B = BC / 2;
+    constexpr int LP = BC + 1;
+    constexpr int CH = 32;
+    constexpr int TS = BC / 8;   // Gram sub-tile per thread (8 × 8 threads per wave)
+    constexpr int NP = BC / 2;   // rotations per inner round
+    __shared__ double2 sG[BC][LP];
+    __shared__ double2 sQ[BC][LP];
+    __shared__ double2 sX[BC][CH + 1];
+    __shared__ double sRot[NP][6];
+    __shared__ int sPair[NP][2];
+    __shared__ int sCol[BC];
+    __shared__ int sFlag;
+
+    const int tid = threadIdx.x;
+    const int2 tt = tiles[blockIdx.x];
+    const JacItem it = items[tt.x];
+    if (*((volatile int*)it.conv)) return;
+    const int R = it.nb - 1;
+    const int r = round % R;
+    int bi, bj;
+    rr_pair(r, tt.y, it.nb, bi, bj);
+    if (tid < BC) {
+        const int c = (tid < B) ? bi * B + tid : bj * B + (tid - B);
+        sCol[tid] = (c < it.q) ? c : -1;
+    }
+    __syncthreads();
+
+    const int p = it.p;
+    // ---------------- Gram matrix ----------------
+    const int lane = tid & 63, wv = tid >> 6;
+    const int ti = lane & 7, tj = lane >> 3;
+    double2 acc[TS][TS];
+#pragma unroll
+    for (int a = 0; a < TS; ++a)
+#pragma unroll
+        for (int b = 0; b < TS; ++b) acc[a][b] = cz();
+
+    for (int row0 = 0; row0 < p; row0 += CH) {
+        for (int e = tid; e < BC * CH; e += 256) {
+            const int c = e / CH, rr = e % CH;
+            const int gc = sCol[c], grow = row0 + rr;
+            sX[c][rr] = (gc >= 0 && grow < p) ? it.x[grow + (size_t)p * gc] : cz();
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int rr = wv; rr < CH; rr += 4) {
+            double2 xi[TS], xj[TS];
+#pragma unroll
+            for (int a = 0; a < TS; ++a) {
+                xi[a] = sX[ti * TS + a][rr];
+                xj[a] = sX[tj * TS + a][rr];
+            }
+#pragma unroll
+            for (int a = 0; a < TS; ++a)
+#pragma unroll
+                for (int b = 0; b < TS; ++b) acc[a][b] = cfmac(xi[a], xj[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int a = 0; a < TS; ++a)
+#pragma unroll
+                for (int b = 0; b < TS; ++b) {
+                    double2& g = sG[ti * TS + a][tj * TS + b];
+                    g = (w == 0) ? acc[a][b] : cadd(g, acc[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+
+    // ---------------- convergence test ----------------
+    if (tid == 0) sFlag = 0;
+    __syncthreads();
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int i = e / BC, j = e % BC;
+        if (i < j) {
+            const double gii = sG[i][i].x, gjj = sG[j][j].x;
+            const double2 g = sG[i][j];
+            if (gii > 0.0 && gjj > 0.0 && sqrt(cabs2(g)) > tol * sqrt(gii) * sqrt(gjj)) sFlag = 1;
+        }
+    }
+    __syncthreads();
+    if (!sFlag) return;
+    if (tid == 0) atomicAdd(it.rot, 1);
+
+    // ---------------- EVD of G by cyclic Jacobi ----------------
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int i = e / BC, j = e % BC;
+        sQ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        if (tid == 0) sFlag = 0;
+        __syncthreads();
+#pragma unroll 1
+        for (int rnd = 0; rnd < BC - 1; ++rnd) {
+            if (tid < NP) {
+                int a, b;
+                rr_pair(rnd, tid, BC, a, b);
+                const double gaa = sG[a][a].x, gbb = sG[b][b].x;
+                const double2 gab = sG[a][b];
+                const double mag = sqrt(cabs2(gab));
+                double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = gaa, nbb = gbb;
+                if (gaa > 0.0 && gbb > 0.0 && mag > tol * sqrt(gaa) * sqrt(gbb)) {
+                    const double zeta = (gbb - gaa) / (2.0 * mag);
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = c * t;
+                    ex = gab.x / mag;
+                    ey = gab.y / mag;
+                    naa = gaa - t * mag;
+                    nbb = gbb + t * mag;
+                    sFlag = 1;
+                }
+                sRot[tid][0] = c; sRot[tid][1] = s; sRot[tid][2] = ex; sRot[tid][3] = ey;
+                sRot[tid][4] = naa; sRot[tid][5] = nbb;
+                sPair[tid][0] = a; sPair[tid][1] = b;
+            }
+            __syncthreads();
+            // columns of G and Q:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
+#pragma unroll 1
+            for (int e = tid; e < 2 * NP * BC; e += 256) {
+                const int mat = e / (NP * BC), rem = e % (NP * BC);
+                const int kp = rem / BC, row = rem % BC;
+                const double s = sRot[kp][1];
+                if (s == 0.0) continue;
+                const double c = sRot[kp][0];
+                const double2 ev = make_double2(sRot[kp][2], sRot[kp][3]);
+                const int a = sPair[kp][0], b = sPair[kp][1];
+                double2 (*M)[LP] = mat ? sQ : sG;
+                const double2 xa = M[row][a], xb = M[row][b];
+                M[row][a] = csub(cscale(xa, c), cscale(cmul(cconj(ev), xb), s));
+                M[row][b] = cadd(cscale(cmul(ev, xa), s), cscale(xb, c));
+            }
+            __syncthreads();
+            // rows of G:  y_a' = c y_a - s e y_b ;  y_b' = s conj(e) y_a + c y_b
+#pragma unroll 1
+            for (int e = tid; e < NP * BC; e += 256) {
+                const int kp = e / BC, col = e % BC;
+                const double s = sRot[kp][1];
+                if (s == 0.0) continue;
+                const double c = sRot[kp][0];
+                const double2 ev = make_double2(sRot[kp][2], sRot[kp][3]);
+                const int a = sPair[kp][0], b = sPair[kp][1];
+                if (col == a) {
+                    sG[a][a] = make_double2(sRot[kp][4], 0.0);
+                    sG[b][a] = cz();
+                } else if (col == b) {
+                    sG[a][b] = cz();
+                    sG[b][b] = make_double2(sRot[kp][5], 0.0);
+                } else {
+                    const double2 ya = sG[a][col], yb = sG[b][col];
+                    sG[a][col] = csub(cscale(ya, c), cscale(cmul(ev, yb), s));
+                    sG[b][col] = cadd(cscale(cmul(cconj(ev), ya), s), cscale(yb, c));
+                }
+            }
+            __syncthreads();
+        }
+        const int any = sFlag;
+        __syncthreads();
+        if (!any) break;
+    }
+
+    // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
+    constexpr int JG = BC / 8;  // output columns per thread
+    const int rr = tid & 31, jg = tid >> 5;
+#pragma unroll 1
+    for (int mat = 0; mat < 2; ++mat) {
+        double2* base = mat ? it.v : it.x;
+        const int rows = mat ? it.q : p;
+        for (int row0 = 0; row0 < rows; row0 += CH) {
+            for (int e = tid; e < BC * CH; e += 256) {
+                const int c = e / CH, r2 = e % CH;
+                const int gc = sCol[c], grow = row0 + r2;
+                sX[c][r2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
+            }
+            __syncthreads();
+            double2 o[JG];
+#pragma unroll
+            for (int a = 0; a < JG; ++a) o[a] = cz();
+#pragma unroll 4
+            for (int i = 0; i < BC; ++i) {
+                const double2 xv = sX[i][rr];
+#pragma unroll
+                for (int a = 0; a < JG; ++a) o[a] = cfma(xv, sQ[i][jg * JG + a], o[a]);
+            }
+            const int grow = row0 + rr;
+#pragma unroll
+            for (int a = 0; a < JG; ++a) {
+                const int gc = sCol[jg * JG + a];
+                if (gc >= 0 && grow < rows) base[grow + (size_t)rows * gc] = o[a];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
+                         double tol, hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (block == 8)
+        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol);
+    else
+        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol);
+}
+
+// V <- I for every item (grid: items × column tiles)
+__global__ void k_set_identity(const JacItem* __restrict__ items, const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const JacItem it = items[tt.x];
+    const int col = tt.y;
+    if (col >= it.q) return;
+    for (int r = threadIdx.x; r < it.q; r += blockDim.x)
+        it.v[r + (size_t)it.q * col] = make_double2(r == col ? 1.0 : 0.0, 0.0);
+    if (threadIdx.x == 0) { *it.rot = 0; *it.conv = 0; }
+}
+void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_set_identity<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
+// After a window of rounds: an item with no rotation is converged.  `remaining`
+// receives the number of items still rotating.
+__global__ void k_conv_update(const JacItem* __restrict__ items, int count, int* remaining) {
+    __shared__ int sRem;
+    if (threadIdx.x == 0) sRem = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const JacItem it = items[i];
+        if (*it.conv == 0) {
+            if (*it.rot == 0) *it.conv = 1;
+            else atomicAdd(&sRem, 1);
+        }
+        *it.rot = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *remaining = sRem;
+}
+void launch_conv_update(const JacItem* d_items, int count, int* remaining, hipStream_t s) {
+    k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining);
+}
+
+// ---------------------------------------------------------------------------
+// singular values: σ_j = ||X[:, j]||  (one wave per column)
+// ---------------------------------------------------------------------------
+__global__ void k_colnorms(const SelectItem* __restrict__ items, const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const SelectItem it = items[tt.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int col = tt.y * 4 + wv;
+    if (col >= it.q) return;
+    const double2* x = it.x + (size_t)it.p * col;
+    double s = 0.0;
+    for (int r = lane; r < it.p; r += 64) s += cabs2(x[r]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) it.sigma_sorted[col] = sqrt(s);
+}
+
+// sort descending, `truncate!` rule, renormalisation, discarded weight
+constexpr int SELECT_MAXQ = 4096;
+__global__ __launch_bounds__(1024) void k_select(const SelectItem* __restrict__ items) {
+    __shared__ double sS[SELECT_MAXQ];
+    __shared__ int sI[SELECT_MAXQ];
+    __shared__ double sRed[32];
+    __shared__ int sKept;
+    const SelectItem it = items[blockIdx.x];
+    const int tid = threadIdx.x, q = it.q;
+    int n2 = 1;
+    while (n2 < q) n2 <<= 1;
+    for (int i = tid; i < n2; i += blockDim.x) {
+        sS[i] = (i < q) ? it.sigma_sorted[i] : -1.0;
+        sI[i] = (i < q) ? i : 0x7fffffff;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < n2; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const double si = sS[i], sj = sS[ixj];
+                    const int ii = sI[i], ij = sI[ixj];
+                    // "i before ixj" in descending order
+                    const bool i_first = (si > sj) || (si == sj && ii < ij);
+                    const bool desc = ((i & k) == 0);
+                    if (desc != i_first) {
+                        sS[i] = sj; sS[ixj] = si;
+                        sI[i] = ij; sI[ixj] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        int lim = q;
+        int kept = q;
+        if (it.truncate) {
+            if (it.maxdim >= 0 && it.maxdim < lim) lim = it.maxdim;
+            kept = 0;
+            while (kept < lim && fabs(sS[kept]) > it.threshold) ++kept;
+        }
+        sKept = kept;
+    }
+    __syncthreads();
+    const int kept = sKept;
+    // discarded weight and kept norm
+    double dsc = 0.0, kn = 0.0;
+    for (int i = tid; i < q; i += blockDim.x) {
+        const double v = sS[i] * sS[i];
+        if (i < kept) kn += v; else dsc += v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dsc += __shfl_xor(dsc, off, 64);
+        kn += __shfl_xor(kn, off, 64);
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) { sRed[wv] = dsc; sRed[16 + wv] = kn; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a += sRed[w]; b += sRed[16 + w]; }
+        sRed[0] = a; sRed[16] = b;
+        *it.kept = kept;
+        *it.discarded = sqrt(a);
+    }
+    __syncthreads();
+    const double scale = (it.renormalize && sRed[16] > 0.0) ? 1.0 / sqrt(sRed[16]) : 1.0;
+    for (int i = tid; i < q; i += blockDim.x) {
+        it.sigma_sorted[i] = sS[i];
+        it.perm[i] = sI[i];
+        if (i < kept) it.lam_out[i] = sS[i] * scale;
+    }
+}
+
+void launch_select_items(const SelectItem* d_items, const int2* d_norm_tiles, int n_norm_tiles, int count,
+                         hipStream_t s) {
+    if (n_norm_tiles > 0) k_colnorms<<<n_norm_tiles, 256, 0, s>>>(d_items, d_norm_tiles);
+    if (count > 0) k_select<<<count, 1024, 0, s>>>(d_items);
+}
+
+// ---------------------------------------------------------------------------
+// write-back of kept columns (32 × 32 tiles: j = output column, c = row of src)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ items, const int2* __restrict__ tiles) {
+    __shared__ double2 sT[32][33];
+    const int2 tt = tiles[blockIdx.x];
+    const GatherItem g = items[tt.x];
+    const int kept = *g.kept;
+    const int ntc = (g.rows + 31) / 32;
+    const int j0 = (tt.y / ntc) * 32, c0 = (tt.y % ntc) * 32;
+    if (j0 >= kept) return;
+    const int tid = threadIdx.x;
+    const int cc = tid & 31;
+    for (int s = 0; s < 4; ++s) {
+        const int jj = (tid >> 5) + 8 * s;
+        const int j = j0 + jj, c = c0 + cc;
+        double2 v = cz();
+        if (j < kept && c < g.rows) {
+            const int src_col = g.perm[j];
+            v = g.src[c + (size_t)g.ld_src * src_col];
+            double f = 1.0;
+            if (g.divide_sigma) {
+                const double sg = g.sigma_sorted[j];
+                f = sg > 0.0 ? 1.0 / sg : 0.0;
+            }
+            if (g.rowscale) f *= g.rowscale[(c / g.rowscale_div) % g.rowscale_mod];
+            v = cscale(v, f);
+        }
+        if (!g.conj_transpose) {
+            if (j < kept && c < g.rows) g.out[c + (size_t)g.rows * j] = v;
+        } else {
+            sT[jj][cc] = cconj(v);
+        }
+    }
+    if (g.conj_transpose) {
+        __syncthreads();
+        for (int s = 0; s < 4; ++s) {
+            const int c = c0 + (tid >> 5) + 8 * s;
+            const int j = j0 + cc;
+            if (j < kept && c < g.rows) g.out[j + (size_t)kept * c] = sT[cc][(tid >> 5) + 8 * s];
+        }
+    }
+}
+void launch_gather_items(const GatherItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_gather<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
+}  // namespace qh

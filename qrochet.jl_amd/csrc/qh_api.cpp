@@ -1,0 +1,949 @@
+// C ABI of libqrochet_hip.so: Qrochet's MPS update path on one MI355X.
+// Each entry point restates one method of src/Ansatz/Chain.jl (cited per
+// function) on device-resident tensors; see include/qrochet_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "qh_runtime.h"
+
+using namespace qh;
+
+namespace qh {
+void launch_apply_op(cplx* t, int left, int dim, int right, const cplx* op, hipStream_t s);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(qh_context* c, F&& f) {
+    try {
+        if (c) QH_HIP(hipSetDevice(c->device));
+        f();
+        if (c) c->err.clear();
+        return QH_OK;
+    } catch (const Error& e) {
+        (c ? c->err : g_err) = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        (c ? c->err : g_err) = "host allocation failed";
+        return QH_ERR_OOM;
+    } catch (const std::exception& e) {
+        (c ? c->err : g_err) = e.what();
+        return QH_ERR_HIP;
+    }
+}
+
+using zc = std::complex<double>;
+
+void check_site(const qh_mps* m, int site) {
+    if (site < 1 || site > m->n) throw Error(QH_ERR_ASSERTION, "Site " + std::to_string(site) + " not found");
+}
+
+void check_bond(const qh_mps* m, int bond) {
+    if (bond < 1 || bond > m->n - 1) throw Error(QH_ERR_ARGUMENT, "Invalid bond (" + std::to_string(bond) + ", " +
+                                                                       std::to_string(bond + 1) + ")");
+}
+
+cplx* dcopy(qh_context* c, const cplx* src, size_t n) {
+    cplx* p = dalloc<cplx>(c, n);
+    if (n) QH_HIP(hipMemcpyAsync(p, src, n * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    return p;
+}
+double* dcopy_d(qh_context* c, const double* src, size_t n) {
+    double* p = dalloc<double>(c, n);
+    if (n) QH_HIP(hipMemcpyAsync(p, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// evolve! for 2-site gates, batched over independent bonds
+// (Chain.jl:606-661 with contract_2sitewf! :669-685 and unpack_2sitewf! :693-722)
+// ---------------------------------------------------------------------------
+void normalize_core(qh_context* c, qh_mps* m, int root);
+
+void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds, const double* gates,
+                  double threshold, int maxdim, unsigned flags, int* kept_out, double* disc_out) {
+    if (count <= 0) return;
+    if (!ms || !bonds || !gates) throw Error(QH_ERR_ARGUMENT, "null argument");
+    const unsigned prec = flags & QH_PREC_MASK;
+    if (prec != QH_PREC_FP64) throw Error(QH_ERR_ARGUMENT, "requested contraction precision is not available");
+    const bool canonical = flags & QH_CANONICAL;
+    const bool renorm = flags & QH_RENORMALIZE;
+    const bool trunc = !std::isnan(threshold) || maxdim >= 0;
+    if (maxdim == 0) throw Error(QH_ERR_ARGUMENT, "maxdim must be positive");
+    const double thr = std::isnan(threshold) ? 1e-16 : threshold;
+    const int d = ms[0] ? ms[0]->d : 0;
+    for (int i = 0; i < count; ++i) {
+        check_mps(c, ms[i]);
+        check_bond(ms[i], bonds[i]);
+        if (ms[i]->d != d) throw Error(QH_ERR_ARGUMENT, "all MPS of a batch must share the physical dimension");
+        for (int j = 0; j < i; ++j)
+            if (ms[j] == ms[i] && std::abs(bonds[j] - bonds[i]) < 2)
+                throw Error(QH_ERR_ARGUMENT, "updates of one batch must act on disjoint sites");
+    }
+    const int d4 = d * d * d * d;
+    std::vector<cplx> hg((size_t)count * d4);
+    std::memcpy(hg.data(), gates, hg.size() * sizeof(cplx));
+    const cplx* dg = upload(c, hg);
+
+    std::vector<ThetaItem> th(count);
+    std::vector<SvdTask> tasks(count);
+    std::vector<cplx*> xbuf(count), nl(count), nr(count);
+    std::vector<double*> nlam(count), pl(count, nullptr), pr(count, nullptr);
+    for (int i = 0; i < count; ++i) {
+        qh_mps* m = ms[i];
+        const int b = bonds[i];
+        const int chil = m->chi[b - 1], chim = m->chi[b], chir = m->chi[b + 1];
+        const int mr = d * chil, mc = d * chir;
+        xbuf[i] = dalloc<cplx>(c, (size_t)mr * mc);
+        ThetaItem& t = th[i];
+        t = ThetaItem{};
+        t.al = m->site[b - 1];
+        t.ar = m->site[b];
+        t.lam = m->lam[b - 1];
+        t.rs = (canonical && b >= 2) ? m->lam[b - 2] : nullptr;
+        t.cs = (canonical && b + 1 <= m->n - 1) ? m->lam[b] : nullptr;
+        t.gate = dg + (size_t)i * d4;
+        t.x = xbuf[i];
+        t.chil = chil; t.chim = chim; t.chir = chir; t.d = d;
+        t.transposed = mr < mc ? 1 : 0;
+        if (t.rs) { pl[i] = dalloc<double>(c, chil); launch_pinv(t.rs, chil, 1e-32, pl[i], c->stream); }
+        if (t.cs) { pr[i] = dalloc<double>(c, chir); launch_pinv(t.cs, chir, 1e-32, pr[i], c->stream); }
+        SvdTask& s = tasks[i];
+        s.x = xbuf[i];
+        s.mr = mr; s.mc = mc;
+        s.truncate = trunc ? 1 : 0;
+        s.threshold = thr;
+        s.maxdim = maxdim;
+        s.renormalize = (canonical && renorm && trunc) ? 1 : 0;
+        const int kmax = s.kmax();
+        nl[i] = dalloc<cplx>(c, (size_t)chil * d * kmax);
+        nr[i] = dalloc<cplx>(c, (size_t)kmax * d * chir);
+        nlam[i] = dalloc<double>(c, kmax);
+        SvdOut u;
+        u.factor = 0; u.conj_transpose = 0; u.rowscale = pl[i]; u.rowscale_mod = chil; u.rowscale_div = 1;
+        u.dst = nl[i];
+        SvdOut v;
+        v.factor = 1; v.conj_transpose = 1; v.rowscale = pr[i]; v.rowscale_mod = chir; v.rowscale_div = d;
+        v.dst = nr[i];
+        s.outs = {u, v};
+        s.lam_dst = nlam[i];
+    }
+    run_theta(c, th);
+    run_svd_tasks(c, tasks);
+    for (int i = 0; i < count; ++i) {
+        qh_mps* m = ms[i];
+        const int b = bonds[i];
+        replace_site(m, b, nl[i]);
+        replace_site(m, b + 1, nr[i]);
+        replace_lam(m, b, nlam[i]);
+        m->chi[b] = tasks[i].kept;
+        dfree(c, xbuf[i]);
+        dfree(c, pl[i]);
+        dfree(c, pr[i]);
+        if (kept_out) kept_out[i] = tasks[i].kept;
+        if (disc_out) disc_out[i] = tasks[i].discarded;
+    }
+    if (renorm && !canonical && trunc)
+        for (int i = 0; i < count; ++i) normalize_core(c, ms[i], bonds[i]);
+}
+
+// ---------------------------------------------------------------------------
+// factorisations of one site (Chain.jl:339-376)
+// ---------------------------------------------------------------------------
+void canonize_site_core(qh_context* c, qh_mps* m, int site, int direction, int method) {
+    check_site(m, site);
+    if (direction == QH_LEFT) {
+        if (site == 1) throw Error(QH_ERR_ARGUMENT, "Cannot right-canonize left-most tensor");
+    } else if (direction == QH_RIGHT) {
+        if (site == m->n) throw Error(QH_ERR_ARGUMENT, "Cannot left-canonize right-most tensor");
+    } else {
+        throw Error(QH_ERR_ARGUMENT, "Unknown direction");
+    }
+    if (method != QH_METHOD_QR && method != QH_METHOD_SVD) throw Error(QH_ERR_ARGUMENT, "Unknown factorization method");
+    const int d = m->d;
+    const int chil = m->chi[site - 1], chir = m->chi[site];
+    // logical matrix: RIGHT -> A as (χl d) × χr ;  LEFT -> A as χl × (d χr)
+    const int mr = direction == QH_RIGHT ? chil * d : chil;
+    const int mc = direction == QH_RIGHT ? chir : d * chir;
+    const int k = std::min(mr, mc);
+    const cplx* a = m->site[site - 1];
+    cplx* x = nullptr;
+    if (mr >= mc) x = dcopy(c, a, (size_t)mr * mc);
+    else {
+        x = dalloc<cplx>(c, (size_t)mr * mc);
+        launch_conj_transpose(a, mr, mc, x, c->stream);
+    }
+    double* sig = dalloc<double>(c, k);
+    SvdTask t;
+    t.x = x; t.mr = mr; t.mc = mc;
+    t.truncate = 0;
+    t.lam_dst = sig;
+    cplx* na = nullptr;   // new tensor at site
+    cplx* f2 = nullptr;   // factor passed to the neighbour
+    if (direction == QH_RIGHT) {
+        na = dalloc<cplx>(c, (size_t)mr * k);   // U (χl d × k)
+        f2 = dalloc<cplx>(c, (size_t)k * mc);   // V^H (k × χr)
+        SvdOut u; u.factor = 0; u.dst = na;
+        SvdOut v; v.factor = 1; v.conj_transpose = 1; v.dst = f2;
+        t.outs = {u, v};
+    } else {
+        na = dalloc<cplx>(c, (size_t)k * mc);   // V^H (k × d χr)
+        f2 = dalloc<cplx>(c, (size_t)mr * k);   // U (χl × k)
+        SvdOut v; v.factor = 1; v.conj_transpose = 1; v.dst = na;
+        SvdOut u; u.factor = 0; u.dst = f2;
+        t.outs = {v, u};
+    }
+    std::vector<SvdTask> tasks{t};
+    run_svd_tasks(c, tasks);
+    dfree(c, x);
+    if (direction == QH_RIGHT) {
+        // A_{i+1} <- (S) V^H · Λ_old · A_{i+1}
+        const int nb = site + 1;
+        const int n2 = d * m->chi[nb];
+        cplx* nn = dalloc<cplx>(c, (size_t)k * n2);
+        std::vector<GemmItem> g{gemm_item(f2, 0, k, m->site[nb - 1], 0, chir, nn, k, k, n2, chir)};
+        g[0].dk = m->lam[site - 1];
+        g[0].dk_mod = chir;
+        if (method == QH_METHOD_QR) g[0].rowscale = sig;
+        run_gemm(c, g);
+        replace_site(m, site, na);
+        replace_site(m, nb, nn);
+        if (method == QH_METHOD_SVD) replace_lam(m, site, sig);
+        else { replace_lam(m, site, nullptr); dfree(c, sig); }
+        m->chi[site] = k;
+    } else {
+        // A_{i-1} <- A_{i-1} · Λ_old · U (S)
+        const int pb = site - 1;
+        const int m2 = m->chi[pb - 1] * d;
+        cplx* nn = dalloc<cplx>(c, (size_t)m2 * k);
+        std::vector<GemmItem> g{gemm_item(m->site[pb - 1], 0, m2, f2, 0, chil, nn, m2, m2, k, chil)};
+        g[0].dk = m->lam[site - 2];
+        g[0].dk_mod = chil;
+        if (method == QH_METHOD_QR) g[0].colscale = sig;
+        run_gemm(c, g);
+        replace_site(m, site, na);
+        replace_site(m, pb, nn);
+        if (method == QH_METHOD_SVD) replace_lam(m, pb, sig);
+        else { replace_lam(m, pb, nullptr); dfree(c, sig); }
+        m->chi[site - 1] = k;
+    }
+    dfree(c, f2);
+}
+
+// Chain.jl:390-422
+void truncate_core(qh_context* c, qh_mps* m, int bond, double threshold, int maxdim) {
+    check_bond(m, bond);
+    if (!m->lam[bond - 1]) throw Error(QH_ERR_MISSING_SCHMIDT, "Can't access the spectrum on bond (" +
+                                                                   std::to_string(bond) + ", " +
+                                                                   std::to_string(bond + 1) + ")");
+    if (maxdim == 0) throw Error(QH_ERR_ARGUMENT, "maxdim must be positive");
+    const int chi = m->chi[bond];
+    std::vector<double> lam(chi);
+    readback(c, lam.data(), m->lam[bond - 1], chi * sizeof(double));
+    const int lim = maxdim >= 0 ? std::min(chi, maxdim) : chi;
+    const double thr = std::isnan(threshold) ? 1e-16 : threshold;
+    std::vector<int> keep;
+    std::vector<double> nl;
+    for (int i = 0; i < lim; ++i)
+        if (std::fabs(lam[i]) > thr) { keep.push_back(i); nl.push_back(lam[i]); }
+    const int kp = (int)keep.size();
+    if (kp == chi) return;
+    const int d = m->d;
+    const int chil = m->chi[bond - 1], chir = m->chi[bond + 1];
+    std::vector<int> hk{kp};
+    const int* dperm = upload(c, keep.empty() ? std::vector<int>{0} : keep);
+    const int* dkept = upload(c, hk);
+    cplx* al = dalloc<cplx>(c, (size_t)chil * d * kp);
+    cplx* ar = dalloc<cplx>(c, (size_t)kp * d * chir);
+    cplx* tmp = dalloc<cplx>(c, (size_t)chi * d * chir);
+    launch_conj_transpose(m->site[bond], chi, d * chir, tmp, c->stream);  // (d χr) × χ
+    std::vector<GatherItem> g(2);
+    g[0] = GatherItem{};
+    g[0].src = m->site[bond - 1]; g[0].ld_src = chil * d; g[0].rows = chil * d;
+    g[0].perm = dperm; g[0].kept = dkept; g[0].out = al; g[0].max_cols = kp;
+    g[0].rowscale_mod = 1; g[0].rowscale_div = 1;
+    g[1] = GatherItem{};
+    g[1].src = tmp; g[1].ld_src = d * chir; g[1].rows = d * chir;
+    g[1].perm = dperm; g[1].kept = dkept; g[1].out = ar; g[1].conj_transpose = 1; g[1].max_cols = kp;
+    g[1].rowscale_mod = 1; g[1].rowscale_div = 1;
+    run_gather(c, g);
+    double* dl = dalloc<double>(c, kp);
+    if (kp) {
+        const double* src = upload(c, nl);
+        QH_HIP(hipMemcpyAsync(dl, src, kp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    replace_site(m, bond, al);
+    replace_site(m, bond + 1, ar);
+    replace_lam(m, bond, dl);
+    m->chi[bond] = kp;
+    dfree(c, tmp);
+    ctx_sync(c);
+}
+
+// Chain.jl:309-333
+void contract_between_core(qh_context* c, qh_mps* m, int s1, int s2, int direction, int del) {
+    check_site(m, s1);
+    check_site(m, s2);
+    if (s1 == s2) throw Error(QH_ERR_ASSERTION, "Sites must be different");
+    if (std::abs(s1 - s2) != 1) return;
+    const int bond = std::min(s1, s2);
+    double* lam = m->lam[bond - 1];
+    if (!lam) return;
+    int target, mode;
+    if (direction == QH_RIGHT) { target = s2; mode = s2 > s1 ? 0 : 2; }
+    else if (direction == QH_LEFT) { target = s1; mode = s2 > s1 ? 2 : 0; }
+    else throw Error(QH_ERR_ARGUMENT, "Unknown direction");
+    launch_scale_bond(m->site[target - 1], m->chi[target - 1], m->d, m->chi[target], lam, mode, 0, 0.0, c->stream);
+    if (del) replace_lam(m, bond, nullptr);
+}
+
+// Chain.jl:469-497
+void canonize_core(qh_context* c, qh_mps* m) {
+    const int n = m->n;
+    for (int i = n; i >= 2; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
+    std::vector<double*> lams(n - 1, nullptr);
+    for (int i = 1; i <= n - 1; ++i) {
+        canonize_site_core(c, m, i, QH_RIGHT, QH_METHOD_SVD);
+        lams[i - 1] = m->lam[i - 1];
+        m->lam[i - 1] = nullptr;
+        launch_scale_bond(m->site[i], m->chi[i], m->d, m->chi[i + 1], lams[i - 1], 0, 0, 0.0, c->stream);
+    }
+    for (int i = 2; i <= n; ++i) {
+        launch_scale_bond(m->site[i - 1], m->chi[i - 1], m->d, m->chi[i], lams[i - 2], 0, 1, 1e-64, c->stream);
+        m->lam[i - 2] = lams[i - 2];
+    }
+    ctx_sync(c);
+}
+
+// Chain.jl:509-524
+void mixed_canonize_core(qh_context* c, qh_mps* m, int center) {
+    check_site(m, center);
+    for (int i = 1; i <= center - 1; ++i) canonize_site_core(c, m, i, QH_RIGHT, QH_METHOD_QR);
+    for (int i = m->n; i >= center + 1; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
+    canonize_site_core(c, m, center, QH_LEFT, QH_METHOD_SVD);
+}
+
+// Chain.jl:532-536
+void normalize_core(qh_context* c, qh_mps* m, int root) {
+    mixed_canonize_core(c, m, root);
+    const int bond = root - 1;
+    const int chi = m->chi[bond];
+    std::vector<double> lam(chi);
+    readback(c, lam.data(), m->lam[bond - 1], chi * sizeof(double));
+    double s = 0.0;
+    for (double v : lam) s += v * v;
+    s = std::sqrt(s);
+    for (double& v : lam) v /= s;
+    const double* src = upload(c, lam);
+    QH_HIP(hipMemcpyAsync(m->lam[bond - 1], src, chi * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    ctx_sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// environments (full-network contraction of Chain.jl:724-748, Ansatz.jl:107-109)
+// ---------------------------------------------------------------------------
+struct Cluster {
+    int start = 0, width = 0;
+    std::vector<zc> op;  // d^w × d^w, column-major, site `start` fastest
+};
+
+// embed op on [s, s+w) into [S, S+W)
+std::vector<zc> embed(const std::vector<zc>& op, int d, int s, int w, int S, int W) {
+    int D = 1;
+    for (int i = 0; i < W; ++i) D *= d;
+    int dw = 1;
+    for (int i = 0; i < w; ++i) dw *= d;
+    int lo = 1;
+    for (int i = 0; i < s - S; ++i) lo *= d;
+    std::vector<zc> out((size_t)D * D, zc(0.0, 0.0));
+    for (int mu = 0; mu < D; ++mu)
+        for (int nu = 0; nu < D; ++nu) {
+            const int mlo = mu % lo, nlo = nu % lo;
+            const int mhi = mu / (lo * dw), nhi = nu / (lo * dw);
+            if (mlo != nlo || mhi != nhi) continue;
+            const int mm = (mu / lo) % dw, nn = (nu / lo) % dw;
+            out[mu + (size_t)D * nu] = op[mm + (size_t)dw * nn];
+        }
+    return out;
+}
+
+std::vector<zc> matmul(const std::vector<zc>& a, const std::vector<zc>& b, int D) {
+    std::vector<zc> out((size_t)D * D, zc(0.0, 0.0));
+    for (int j = 0; j < D; ++j)
+        for (int k = 0; k < D; ++k) {
+            const zc bk = b[k + (size_t)D * j];
+            if (bk == zc(0.0, 0.0)) continue;
+            for (int i = 0; i < D; ++i) out[i + (size_t)D * j] += a[i + (size_t)D * k] * bk;
+        }
+    return out;
+}
+
+// compose an ordered observable list into disjoint clusters (later ops act after earlier ones)
+std::vector<Cluster> make_clusters(const qh_mps* m, int nobs, const qh_observable* obs) {
+    const int d = m->d;
+    std::vector<Cluster> cl;
+    for (int k = 0; k < nobs; ++k) {
+        const qh_observable& o = obs[k];
+        if (o.nsites != 1 && o.nsites != 2) throw Error(QH_ERR_ARGUMENT, "observables act on 1 or 2 sites");
+        if (!o.op) throw Error(QH_ERR_ARGUMENT, "null observable");
+        if (o.site < 1 || o.site + o.nsites - 1 > m->n)
+            throw Error(QH_ERR_ARGUMENT, "Gate inputs must be a subset of the TN sites");
+        int dw = o.nsites == 1 ? d : d * d;
+        Cluster cur;
+        cur.start = o.site;
+        cur.width = o.nsites;
+        cur.op.resize((size_t)dw * dw);
+        std::memcpy(cur.op.data(), o.op, cur.op.size() * sizeof(zc));
+        // merge with every overlapping cluster
+        int S = cur.start, E = cur.start + cur.width - 1;
+        std::vector<Cluster> keep, over;
+        for (auto& c2 : cl) {
+            const int s2 = c2.start, e2 = c2.start + c2.width - 1;
+            if (e2 < S || s2 > E) keep.push_back(c2);
+            else { over.push_back(c2); }
+        }
+        for (auto& c2 : over) { S = std::min(S, c2.start); E = std::max(E, c2.start + c2.width - 1); }
+        const int W = E - S + 1;
+        int D = 1;
+        for (int i = 0; i < W; ++i) D *= d;
+        if (W > 4 || D > 16) throw Error(QH_ERR_ARGUMENT, "observables overlapping over more than 4 sites");
+        std::vector<zc> prev((size_t)D * D, zc(0.0, 0.0));
+        for (int i = 0; i < D; ++i) prev[i + (size_t)D * i] = 1.0;
+        for (auto& c2 : over) prev = matmul(embed(c2.op, d, c2.start, c2.width, S, W), prev, D);
+        Cluster merged;
+        merged.start = S;
+        merged.width = W;
+        merged.op = matmul(embed(cur.op, d, cur.start, cur.width, S, W), prev, D);
+        keep.push_back(merged);
+        cl.swap(keep);
+    }
+    std::sort(cl.begin(), cl.end(), [](const Cluster& a, const Cluster& b) { return a.start < b.start; });
+    return cl;
+}
+
+int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; ++i) r *= b;
+    return r;
+}
+
+// Advance the left environment E (χ_b × χ_a on bond s-1, without that bond's Λ)
+// through sites s..e with the given clusters; returns the environment on bond e.
+cplx* env_segment(qh_context* c, const qh_mps* ket, const qh_mps* bra, cplx* E, int s, int e,
+                  const std::vector<Cluster>& cls) {
+    const int d = ket->d;
+    size_t ci = 0;
+    while (ci < cls.size() && cls[ci].start < s) ++ci;
+    int i = s;
+    while (i <= e) {
+        const int lb = bra->chi[i - 1];
+        const bool has_cluster = ci < cls.size() && cls[ci].start == i;
+        const int w = has_cluster ? cls[ci].width : 1;
+        // ket: T = E Λa_{i-1} A_i A_{i+1} ...
+        cplx* T = dalloc<cplx>(c, (size_t)lb * d * ket->chi[i]);
+        {
+            std::vector<GemmItem> g{gemm_item(E, 0, lb, ket->site[i - 1], 0, ket->chi[i - 1], T, lb, lb, d * ket->chi[i],
+                                              ket->chi[i - 1])};
+            if (i >= 2) { g[0].dk = ket->lam[i - 2]; g[0].dk_mod = ket->chi[i - 1]; }
+            run_gemm(c, g);
+        }
+        dfree(c, E);
+        for (int s2 = 1; s2 < w; ++s2) {
+            const int j = i + s2;
+            const int rows = lb * ipow(d, s2);
+            cplx* T2 = dalloc<cplx>(c, (size_t)rows * d * ket->chi[j]);
+            std::vector<GemmItem> g{gemm_item(T, 0, rows, ket->site[j - 1], 0, ket->chi[j - 1], T2, rows, rows,
+                                              d * ket->chi[j], ket->chi[j - 1])};
+            g[0].dk = ket->lam[j - 2];
+            g[0].dk_mod = ket->chi[j - 1];
+            run_gemm(c, g);
+            dfree(c, T);
+            T = T2;
+        }
+        const int ra = ket->chi[i + w - 1];
+        if (has_cluster) {
+            std::vector<cplx> hop(cls[ci].op.size());
+            std::memcpy(hop.data(), cls[ci].op.data(), hop.size() * sizeof(cplx));
+            const cplx* dop = upload(c, hop);
+            launch_apply_op(T, lb, ipow(d, w), ra, dop, c->stream);
+        }
+        // bra: peel one site at a time
+        int curl = lb;
+        for (int s2 = 0; s2 < w; ++s2) {
+            const int j = i + s2;
+            const int rest = ipow(d, w - 1 - s2) * ra;
+            const int rb = bra->chi[j];
+            cplx* W = dalloc<cplx>(c, (size_t)rb * rest);
+            std::vector<GemmItem> g{gemm_item(bra->site[j - 1], 1, curl * d, T, 0, curl * d, W, rb, rb, rest, curl * d)};
+            if (j >= 2) { g[0].dk = bra->lam[j - 2]; g[0].dk_mod = curl; }
+            run_gemm(c, g);
+            dfree(c, T);
+            T = W;
+            curl = rb;
+        }
+        E = T;
+        i += w;
+        if (has_cluster) ++ci;
+    }
+    return E;
+}
+
+cplx* unit_env(qh_context* c) {
+    std::vector<cplx> one{make_double2(1.0, 0.0)};
+    const cplx* src = upload(c, one);
+    cplx* e = dalloc<cplx>(c, 1);
+    QH_HIP(hipMemcpyAsync(e, src, sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    return e;
+}
+
+zc full_contraction(qh_context* c, const qh_mps* ket, const qh_mps* bra, const std::vector<Cluster>& cls) {
+    cplx* E = env_segment(c, ket, bra, unit_env(c), 1, ket->n, cls);
+    cplx h;
+    readback(c, &h, E, sizeof(cplx));
+    dfree(c, E);
+    return zc(h.x, h.y);
+}
+
+// right environments R_i (ket × bra on bond i, without Λ_i), i = 0..n
+std::vector<cplx*> right_envs(qh_context* c, const qh_mps* ket, const qh_mps* bra) {
+    const int n = ket->n, d = ket->d;
+    std::vector<cplx*> R(n + 1, nullptr);
+    R[n] = unit_env(c);
+    for (int i = n; i >= 1; --i) {
+        const int la = ket->chi[i - 1], ra = ket->chi[i], lb = bra->chi[i - 1], rb = bra->chi[i];
+        cplx* T = dalloc<cplx>(c, (size_t)la * d * rb);
+        std::vector<GemmItem> g{gemm_item(ket->site[i - 1], 0, la * d, R[i], 0, ra, T, la * d, la * d, rb, ra)};
+        if (i <= n - 1) { g[0].dk = ket->lam[i - 1]; g[0].dk_mod = ra; }
+        run_gemm(c, g);
+        R[i - 1] = dalloc<cplx>(c, (size_t)la * lb);
+        std::vector<GemmItem> g2{gemm_item(T, 0, la, bra->site[i - 1], 1, lb, R[i - 1], la, la, lb, d * rb)};
+        if (i <= n - 1) { g2[0].dk = bra->lam[i - 1]; g2[0].dk_mod = rb; g2[0].dk_div = d; }
+        run_gemm(c, g2);
+        dfree(c, T);
+    }
+    return R;
+}
+
+std::vector<cplx*> left_envs(qh_context* c, const qh_mps* ket, const qh_mps* bra) {
+    const int n = ket->n;
+    std::vector<cplx*> L(n + 1, nullptr);
+    L[0] = unit_env(c);
+    for (int i = 1; i <= n; ++i) {
+        const size_t sz = (size_t)bra->chi[i - 1] * ket->chi[i - 1];
+        cplx* E = dcopy(c, L[i - 1], sz);
+        L[i] = env_segment(c, ket, bra, E, i, i, {});
+    }
+    return L;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int qh_ctx_create(int device, qh_context** out) {
+    if (!out) { g_err = "null output pointer"; return QH_ERR_ARGUMENT; }
+    qh_context* c = nullptr;
+    int rc = guard(nullptr, [&] {
+        int ndev = 0;
+        QH_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw Error(QH_ERR_ARGUMENT, "invalid device ordinal");
+        QH_HIP(hipSetDevice(device));
+        c = new qh_context();
+        c->device = device;
+        QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        c->arena_cap = size_t(4) << 20;
+        QH_HIP(hipHostMalloc((void**)&c->h_arena, c->arena_cap, hipHostMallocDefault));
+        QH_HIP(hipMalloc((void**)&c->d_arena, c->arena_cap));
+    });
+    if (rc != QH_OK) {
+        if (c) {
+            if (c->stream) (void)hipStreamDestroy(c->stream);
+            delete c;
+        }
+        return rc;
+    }
+    *out = c;
+    return QH_OK;
+}
+
+int qh_ctx_destroy(qh_context* c) {
+    if (!c) return QH_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) { c->event_pool.push_back(p.a); c->event_pool.push_back(p.b); }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->h_arena) (void)hipHostFree(c->h_arena);
+    if (c->d_arena) (void)hipFree(c->d_arena);
+    if (c->h_read) (void)hipHostFree(c->h_read);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QH_OK;
+}
+
+int qh_ctx_synchronize(qh_context* c) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { ctx_sync(c); });
+}
+
+const char* qh_last_error(const qh_context* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int qh_ctx_set_timing(qh_context* c, int enable) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        ctx_sync(c);
+        c->timing = enable != 0;
+    });
+}
+
+int qh_ctx_get_timing(qh_context* c, const char* cls, double* ms, int64_t* launches, double* flops, double* bytes) {
+    if (!c || !cls) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        ctx_sync(c);
+        auto it = c->timers.find(cls);
+        TimerClass t = it == c->timers.end() ? TimerClass{} : it->second;
+        if (ms) *ms = t.ms;
+        if (launches) *launches = t.launches;
+        if (flops) *flops = t.flops;
+        if (bytes) *bytes = t.bytes;
+    });
+}
+
+int qh_ctx_reset_timing(qh_context* c) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        ctx_sync(c);
+        c->timers.clear();
+    });
+}
+
+int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        if (block != 0 && block != 8 && block != 16) throw Error(QH_ERR_ARGUMENT, "Jacobi block must be 8 or 16");
+        if (block) c->jac_block = block;
+        if (max_sweeps > 0) c->jac_max_sweeps = max_sweeps;
+    });
+}
+
+int qh_mps_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays,
+                  const double* const* lambdas, qh_mps** out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    qh_mps* m = nullptr;
+    int rc = guard(c, [&] {
+        if (n < 2) throw Error(QH_ERR_ARGUMENT, "a Chain needs at least 2 sites");
+        if (d < 2 || d > 4) throw Error(QH_ERR_ARGUMENT, "physical dimension must be 2, 3 or 4");
+        if (!bond_dims || !arrays) throw Error(QH_ERR_ARGUMENT, "null argument");
+        for (int b = 0; b < n - 1; ++b)
+            if (bond_dims[b] < 1) throw Error(QH_ERR_DIMENSION, "bond dimensions must be positive");
+        m = new qh_mps();
+        m->ctx = c;
+        m->n = n;
+        m->d = d;
+        m->chi.assign(n + 1, 1);
+        for (int b = 1; b <= n - 1; ++b) m->chi[b] = bond_dims[b - 1];
+        m->site.assign(n, nullptr);
+        m->lam.assign(n - 1, nullptr);
+        std::vector<std::vector<cplx>> stage(n);
+        for (int i = 1; i <= n; ++i) {
+            if (!arrays[i - 1]) throw Error(QH_ERR_ARGUMENT, "null site array");
+            const int cl = m->chi[i - 1], cr = m->chi[i];
+            const size_t ne = site_elems(m, i);
+            const cplx* src = reinterpret_cast<const cplx*>(arrays[i - 1]);
+            auto& st = stage[i - 1];
+            st.resize(ne);
+            // (o, l, r) -> (l, o, r)
+            for (int r = 0; r < cr; ++r)
+                for (int l = 0; l < cl; ++l)
+                    for (int o = 0; o < d; ++o)
+                        st[l + (size_t)cl * (o + (size_t)d * r)] = src[o + (size_t)d * (l + (size_t)cl * r)];
+            m->site[i - 1] = dalloc<cplx>(c, ne);
+            QH_HIP(hipMemcpyAsync(m->site[i - 1], st.data(), ne * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+        }
+        if (lambdas) {
+            for (int b = 1; b <= n - 1; ++b) {
+                if (!lambdas[b - 1]) continue;
+                m->lam[b - 1] = dalloc<double>(c, m->chi[b]);
+                QH_HIP(hipMemcpyAsync(m->lam[b - 1], lambdas[b - 1], m->chi[b] * sizeof(double), hipMemcpyHostToDevice,
+                                      c->stream));
+            }
+        }
+        ctx_sync(c);
+    });
+    if (rc != QH_OK) {
+        if (m) {
+            for (auto p : m->site) dfree(c, p);
+            for (auto p : m->lam) dfree(c, p);
+            delete m;
+        }
+        return rc;
+    }
+    *out = m;
+    return QH_OK;
+}
+
+int qh_mps_clone(qh_context* c, const qh_mps* src, qh_mps** out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, src);
+        qh_mps* m = new qh_mps(*src);
+        for (int i = 1; i <= m->n; ++i) m->site[i - 1] = dcopy(c, src->site[i - 1], site_elems(src, i));
+        for (int b = 1; b <= m->n - 1; ++b)
+            m->lam[b - 1] = src->lam[b - 1] ? dcopy_d(c, src->lam[b - 1], src->chi[b]) : nullptr;
+        *out = m;
+    });
+}
+
+int qh_mps_free(qh_mps* m) {
+    if (!m) return QH_OK;
+    qh_context* c = m->ctx;
+    int rc = guard(c, [&] {
+        for (auto p : m->site) dfree(c, p);
+        for (auto p : m->lam) dfree(c, p);
+    });
+    delete m;
+    return rc;
+}
+
+int qh_mps_nsites(const qh_mps* m, int* n, int* d) {
+    if (!m) return QH_ERR_ARGUMENT;
+    if (n) *n = m->n;
+    if (d) *d = m->d;
+    return QH_OK;
+}
+
+int qh_mps_bond_dims(qh_mps* m, int* dims, int* has) {
+    if (!m) return QH_ERR_ARGUMENT;
+    for (int b = 1; b <= m->n - 1; ++b) {
+        if (dims) dims[b - 1] = m->chi[b];
+        if (has) has[b - 1] = m->lam[b - 1] ? 1 : 0;
+    }
+    return QH_OK;
+}
+
+int qh_mps_get_site(qh_mps* m, int site, double* host) {
+    if (!m || !host) return QH_ERR_ARGUMENT;
+    qh_context* c = m->ctx;
+    return guard(c, [&] {
+        check_site(m, site);
+        const int cl = m->chi[site - 1], cr = m->chi[site], d = m->d;
+        const size_t ne = site_elems(m, site);
+        std::vector<cplx> tmp(ne);
+        readback(c, tmp.data(), m->site[site - 1], ne * sizeof(cplx));
+        cplx* dst = reinterpret_cast<cplx*>(host);
+        for (int r = 0; r < cr; ++r)
+            for (int l = 0; l < cl; ++l)
+                for (int o = 0; o < d; ++o)
+                    dst[o + (size_t)d * (l + (size_t)cl * r)] = tmp[l + (size_t)cl * (o + (size_t)d * r)];
+    });
+}
+
+int qh_mps_get_lambda(qh_mps* m, int bond, double* host) {
+    if (!m || !host) return QH_ERR_ARGUMENT;
+    qh_context* c = m->ctx;
+    return guard(c, [&] {
+        check_bond(m, bond);
+        if (!m->lam[bond - 1])
+            throw Error(QH_ERR_MISSING_SCHMIDT, "Can't access the spectrum on bond (" + std::to_string(bond) + ", " +
+                                                    std::to_string(bond + 1) + ")");
+        readback(c, host, m->lam[bond - 1], m->chi[bond] * sizeof(double));
+    });
+}
+
+int qh_mps_footprint(const qh_mps* m, size_t* bytes) {
+    if (!m || !bytes) return QH_ERR_ARGUMENT;
+    size_t b = 0;
+    for (int i = 1; i <= m->n; ++i) b += site_elems(m, i) * sizeof(cplx);
+    for (int k = 1; k <= m->n - 1; ++k)
+        if (m->lam[k - 1]) b += m->chi[k] * sizeof(double);
+    *bytes = b;
+    return QH_OK;
+}
+
+int qh_evolve1(qh_context* c, qh_mps* m, int site, const double* gate) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        if (!gate) throw Error(QH_ERR_ARGUMENT, "null gate");
+        if (site < 1 || site > m->n) throw Error(QH_ERR_ARGUMENT, "Gate inputs must be a subset of the TN sites");
+        const int d = m->d;
+        std::vector<cplx> hg((size_t)d * d);
+        std::memcpy(hg.data(), gate, hg.size() * sizeof(cplx));
+        const cplx* dg = upload(c, hg);
+        Timed tm(c, "apply1");
+        launch_apply_op(m->site[site - 1], m->chi[site - 1], d, m->chi[site], dg, c->stream);
+    });
+}
+
+int qh_evolve2(qh_context* c, qh_mps* m, int bond, const double* gate, double threshold, int maxdim, unsigned flags,
+               int* kept, double* discarded) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { evolve2_core(c, 1, &m, &bond, gate, threshold, maxdim, flags, kept, discarded); });
+}
+
+int qh_evolve2_batch(qh_context* c, int count, qh_mps* const* ms, const int* bonds, const double* gates,
+                     double threshold, int maxdim, unsigned flags, int* kept, double* discarded) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { evolve2_core(c, count, ms, bonds, gates, threshold, maxdim, flags, kept, discarded); });
+}
+
+int qh_truncate(qh_context* c, qh_mps* m, int bond, double threshold, int maxdim) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        truncate_core(c, m, bond, threshold, maxdim);
+    });
+}
+
+int qh_contract_between(qh_context* c, qh_mps* m, int s1, int s2, int direction, int del) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        contract_between_core(c, m, s1, s2, direction, del);
+    });
+}
+
+int qh_canonize_site(qh_context* c, qh_mps* m, int site, int direction, int method) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        canonize_site_core(c, m, site, direction, method);
+    });
+}
+
+int qh_canonize(qh_context* c, qh_mps* m) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        canonize_core(c, m);
+    });
+}
+
+int qh_mixed_canonize(qh_context* c, qh_mps* m, int center) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        mixed_canonize_core(c, m, center);
+    });
+}
+
+int qh_normalize(qh_context* c, qh_mps* m, int root) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        check_site(m, root);
+        normalize_core(c, m, root);
+    });
+}
+
+static void canon_test(qh_context* c, qh_mps* m, int site, double atol, int* result, double* dev, bool left) {
+    check_mps(c, m);
+    check_site(m, site);
+    const int cl = m->chi[site - 1], cr = m->chi[site], d = m->d;
+    const int nn = left ? cr : cl;
+    cplx* g = dalloc<cplx>(c, (size_t)nn * nn);
+    std::vector<GemmItem> gi;
+    if (left) gi.push_back(gemm_item(m->site[site - 1], 1, cl * d, m->site[site - 1], 0, cl * d, g, cr, cr, cr, cl * d));
+    else gi.push_back(gemm_item(m->site[site - 1], 0, cl, m->site[site - 1], 1, cl, g, cl, cl, cl, d * cr));
+    run_gemm(c, gi);
+    double* dd = dalloc<double>(c, 1);
+    launch_gram_dev(g, nn, dd, c->stream);
+    double h = 0.0;
+    readback(c, &h, dd, sizeof(double));
+    dfree(c, g);
+    dfree(c, dd);
+    if (result) *result = (h <= atol) ? 1 : 0;
+    if (dev) *dev = h;
+}
+
+int qh_is_left_canonical(qh_context* c, qh_mps* m, int site, double atol, int* result, double* dev) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { canon_test(c, m, site, atol, result, dev, true); });
+}
+
+int qh_is_right_canonical(qh_context* c, qh_mps* m, int site, double atol, int* result, double* dev) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { canon_test(c, m, site, atol, result, dev, false); });
+}
+
+int qh_expect(qh_context* c, qh_mps* m, int nobs, const qh_observable* obs, double* out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        if (nobs < 0 || (nobs > 0 && !obs)) throw Error(QH_ERR_ARGUMENT, "bad observable list");
+        auto cls = make_clusters(m, nobs, obs);
+        zc v = full_contraction(c, m, m, cls);
+        out[0] = v.real();
+        out[1] = v.imag();
+    });
+}
+
+int qh_expect_many(qh_context* c, qh_mps* m, int nterms, const int* offsets, const qh_observable* obs, double* out) {
+    if (!c || !out || !offsets) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        std::vector<std::vector<Cluster>> terms(nterms);
+        for (int t = 0; t < nterms; ++t) {
+            if (offsets[t + 1] < offsets[t]) throw Error(QH_ERR_ARGUMENT, "bad term offsets");
+            terms[t] = make_clusters(m, offsets[t + 1] - offsets[t], obs + offsets[t]);
+        }
+        auto L = left_envs(c, m, m);
+        auto R = right_envs(c, m, m);
+        cplx* vals = dalloc<cplx>(c, std::max(nterms, 1));
+        for (int t = 0; t < nterms; ++t) {
+            int s = 1, e = 1;
+            if (!terms[t].empty()) {
+                s = terms[t].front().start;
+                e = terms[t].back().start + terms[t].back().width - 1;
+            }
+            cplx* E = dcopy(c, L[s - 1], (size_t)m->chi[s - 1] * m->chi[s - 1]);
+            E = env_segment(c, m, m, E, s, e, terms[t]);
+            const double* lb = e <= m->n - 1 ? m->lam[e - 1] : nullptr;
+            launch_env_close(E, R[e], m->chi[e], m->chi[e], lb, lb, vals + t, c->stream);
+            dfree(c, E);
+        }
+        std::vector<cplx> hv(std::max(nterms, 1));
+        readback(c, hv.data(), vals, hv.size() * sizeof(cplx));
+        std::memcpy(out, hv.data(), (size_t)nterms * sizeof(cplx));
+        dfree(c, vals);
+        for (auto p : L) dfree(c, p);
+        for (auto p : R) dfree(c, p);
+    });
+}
+
+int qh_overlap(qh_context* c, qh_mps* a, qh_mps* b, double* out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, a);
+        check_mps(c, b);
+        if (a->n != b->n || a->d != b->d) throw Error(QH_ERR_ASSERTION, "Ansatzes must have the same sites");
+        zc v = full_contraction(c, a, b, {});
+        out[0] = v.real();
+        out[1] = v.imag();
+    });
+}
+
+int qh_norm(qh_context* c, qh_mps* m, double* out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        check_mps(c, m);
+        zc v = full_contraction(c, m, m, {});
+        *out = std::abs(std::sqrt(v));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Complex FP64 helpers for device code (interleaved re/im in double2).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qh {
+
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// c + a * b
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+    c.x = fma(a.x, b.x, c.x);
+    c.x = fma(-a.y, b.y, c.x);
+    c.y = fma(a.x, b.y, c.y);
+    c.y = fma(a.y, b.x, c.y);
+    return c;
+}
+// c + conj(a) * b
+__device__ __forceinline__ double2 cfmac(double2 a, double2 b, double2 c) {
+    c.x = fma(a.x, b.x, c.x);
+    c.x = fma(a.y, b.y, c.x);
+    c.y = fma(a.x, b.y, c.y);
+    c.y = fma(-a.y, b.x, c.y);
+    return c;
+}
+__device__ __forceinline__ double cabs2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
+
+}  // namespace qh

@@ -1,0 +1,137 @@
+// Internal kernel interface of libqrochet_hip.so (gfx950).
+//
+// Device data layout of one MPS site tensor A[l, o, r]: complex FP64, l fastest,
+// then o, then r (column-major (l, o, r)).  The same memory is the (χl·d) × χr
+// matrix with rows (l, o) and the χl × (d·χr) matrix with columns (o, r), so the
+// two-site contraction and both SVD factors need no permutation.
+//
+// Batched launches take an array of item descriptors and a tile map (int2:
+// item, tile-within-item), both in device memory, so one launch serves items
+// of different shapes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qh {
+
+using cplx = double2;
+
+// ---------------------------------------------------------------------------
+// θ build (two-site contraction + gate), one item per bond update
+// ---------------------------------------------------------------------------
+struct ThetaItem {
+    const cplx* al;      // A_l  (χl, d, χm)
+    const cplx* ar;      // A_r  (χm, d, χr)
+    const double* lam;   // Λ on the bond (χm) or null
+    const double* rs;    // left scale on l (χl) or null  (Vidal mode: Λ_{i-1})
+    const double* cs;    // right scale on r (χr) or null (Vidal mode: Λ_{i+1})
+    const cplx* gate;    // d^4, (out_l, out_r, in_l, in_r) column-major
+    cplx* x;             // output: θ (m × n, ld m) or θ^H (n × m, ld n)
+    int chil, chim, chir, d;
+    int transposed;      // 0: X = θ,  1: X = θ^H
+    int tiles_l, tiles_r;
+};
+// FP64 vector-FMA build; d in {2, 3, 4}.  Tiles cover theta_tile_extent(d)
+// values of l and of r.
+void launch_theta(const ThetaItem* d_items, const int2* d_tiles, int ntiles, int d, hipStream_t s);
+int theta_tile_extent(int d);
+
+// ---------------------------------------------------------------------------
+// batched complex GEMM:  C = alpha · op(A) · diag(dk) · op(B) + beta · C
+// then C rows scaled by rowscale, columns by colscale (nullable)
+// ---------------------------------------------------------------------------
+struct GemmItem {
+    const cplx* a;
+    const cplx* b;
+    cplx* c;
+    const double* dk;        // K scaling, index (k / dk_div) % dk_mod (null = none)
+    const double* rowscale;  // M
+    const double* colscale;  // N
+    cplx alpha, beta;
+    int m, n, k, lda, ldb, ldc;
+    int opa, opb;            // 0 = N, 1 = conjugate transpose
+    int dk_mod, dk_div;
+    int tiles_m, tiles_n;
+};
+void launch_gemm(const GemmItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+constexpr int GEMM_TM = 64, GEMM_TN = 64, GEMM_TK = 16;
+
+// ---------------------------------------------------------------------------
+// batched block one-sided Jacobi SVD
+// ---------------------------------------------------------------------------
+struct JacItem {
+    cplx* x;        // p × q working matrix (ld p), q <= p, columns orthogonalised in place
+    cplx* v;        // q × q accumulated right rotations (ld q)
+    int p, q;
+    int nb;         // column blocks (even >= 2)
+    int* rot;       // rotations performed in the current window (device counter)
+    int* conv;      // 1 once converged (kernels skip the item)
+};
+// block = columns per block (8 or 16); a workgroup handles 2·block columns.
+void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
+                         double tol, hipStream_t s);
+// V <- I, counters cleared (tile map: item, column)
+void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// converged <- (no rotation in the window); *remaining = items still rotating
+void launch_conv_update(const JacItem* d_items, int count, int* remaining, hipStream_t s);
+
+// Finalisation: σ_j = ||X[:, j]||, sort descending, truncation rule of
+// `truncate!`, optional renormalisation, kept count and discarded weight.
+struct SelectItem {
+    const cplx* x;
+    int p, q;
+    double* sigma_sorted;   // q (in: unsorted norms; out: descending)
+    int* perm;              // q (source column of each sorted value)
+    double* lam_out;        // kept Λ (k'), renormalised if requested
+    int* kept;              // device scalar
+    double* discarded;      // device scalar
+    double threshold;       // already defaulted (1e-16 if only maxdim)
+    int maxdim;             // < 0 -> none
+    int truncate;           // 0 -> keep all q
+    int renormalize;
+};
+constexpr int SELECT_MAX_Q = 4096;
+// norm tiles: (item, group of 4 columns); one select workgroup per item
+void launch_select_items(const SelectItem* d_items, const int2* d_norm_tiles, int n_norm_tiles, int count,
+                         hipStream_t s);
+
+// out[r, j] = src[r, perm[j]] * colfac(j) * rowscale[(r / div) % mod] for j < kept
+//   colfac = 1/sigma_sorted[j] if divide_sigma (0 if sigma == 0) else 1
+//   conj_transpose: out is (kept × rows), ld kept, holding the conjugate
+struct GatherItem {
+    const cplx* src;
+    int ld_src;
+    int rows;
+    const int* perm;
+    const double* sigma_sorted;
+    int divide_sigma;
+    const int* kept;
+    const double* rowscale;
+    int rowscale_mod, rowscale_div;
+    cplx* out;
+    int conj_transpose;
+    int max_cols;
+};
+// tile map: (item, tile) over 32 × 32 tiles, tile = jt * ceil(rows/32) + ct
+void launch_gather_items(const GatherItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+// T[l, μ, r] <- Σ_ν Op[μ + D ν] T[l, ν, r], D in {2,3,4,8,9,16}
+void launch_apply_op(cplx* t, int left, int dim, int right, const cplx* op, hipStream_t s);
+// A <- A · f(λ) along l (mode 0) or r (mode 2); f = pinv ? pinv(λ; atol) : λ
+void launch_scale_bond(cplx* a, int chil, int d, int chir, const double* scale, int mode, int pinv,
+                       double atol, hipStream_t s);
+void launch_pinv(const double* lam, int n, double atol, double* out, hipStream_t s);
+// dst (cols × rows) = src^H, src rows × cols (ld rows)
+void launch_conj_transpose(const cplx* src, int rows, int cols, cplx* dst, hipStream_t s);
+// ||G - I||_F
+void launch_gram_dev(const cplx* gram, int n, double* dev_out, hipStream_t s);
+// Σ_{b,a} E[b,a] R[a,b] λb[b] λa[a]
+void launch_env_close(const cplx* e, const cplx* r, int nb, int na, const double* lb, const double* la, cplx* out,
+                      hipStream_t s);
+void launch_fill(double* p, size_t n, double v, hipStream_t s);
+void launch_eye(cplx* p, int n, hipStream_t s);
+
+}  // namespace qh

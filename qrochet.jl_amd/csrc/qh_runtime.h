@@ -1,0 +1,138 @@
+// Host runtime of libqrochet_hip.so: context (stream, descriptor arena, event
+// timing), MPS objects, and the batched building blocks the C ABI composes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qh_kernels.h"
+#include "qrochet_hip.h"
+
+namespace qh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define QH_HIP(x) ::qh::hip_check((x), #x)
+
+struct TimerClass {
+    double ms = 0.0;
+    int64_t launches = 0;
+    double flops = 0.0;
+    double bytes = 0.0;
+};
+
+}  // namespace qh
+
+struct qh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // descriptor arena: pinned host staging + device mirror, reset at every sync
+    char* h_arena = nullptr;
+    char* d_arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
+    // small pinned readback area
+    char* h_read = nullptr;
+    size_t read_cap = 0;
+    // kernel-class timing
+    bool timing = false;
+    std::map<std::string, qh::TimerClass> timers;
+    struct Pending {
+        std::string cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    // Jacobi parameters
+    int jac_block = 16;
+    int jac_max_sweeps = 40;
+};
+
+struct qh_mps {
+    qh_context* ctx = nullptr;
+    int n = 0, d = 0;
+    std::vector<int> chi;            // n+1 entries: chi[0] = chi[n] = 1, chi[b] bond b
+    std::vector<qh::cplx*> site;     // n
+    std::vector<double*> lam;        // n-1 (bond b -> lam[b-1]); null when absent
+};
+
+namespace qh {
+
+// ---- context helpers ----
+void ctx_sync(qh_context* c);
+void* dalloc_bytes(qh_context* c, size_t bytes);
+template <class T>
+T* dalloc(qh_context* c, size_t n) {
+    return static_cast<T*>(dalloc_bytes(c, n * sizeof(T)));
+}
+void dfree(qh_context* c, void* p);
+// copy host bytes to the descriptor arena; returns a device pointer valid until the next sync
+void* arena_upload(qh_context* c, const void* src, size_t bytes);
+template <class T>
+T* upload(qh_context* c, const std::vector<T>& v) {
+    return static_cast<T*>(arena_upload(c, v.data(), v.size() * sizeof(T)));
+}
+// device -> host readback (synchronises)
+void readback(qh_context* c, void* host, const void* dev, size_t bytes);
+
+// timing scope around one launch
+struct Timed {
+    qh_context* c;
+    const char* cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(qh_context* c, const char* cls, double flops = 0.0, double bytes = 0.0);
+    ~Timed();
+};
+
+// ---- batched building blocks ----
+void run_theta(qh_context* c, std::vector<ThetaItem>& items);
+void run_gemm(qh_context* c, std::vector<GemmItem>& items);
+// runs set-identity + sweeps until every item converged; throws on sweep limit
+void run_jacobi(qh_context* c, std::vector<JacItem>& items);
+void run_select(qh_context* c, std::vector<SelectItem>& items);
+void run_gather(qh_context* c, std::vector<GatherItem>& items);
+
+GemmItem gemm_item(const cplx* a, int opa, int lda, const cplx* b, int opb, int ldb, cplx* cc, int ldc, int m, int n,
+                   int k);
+
+// ---- MPS helpers ----
+void check_mps(qh_context* c, const qh_mps* m);
+size_t site_elems(const qh_mps* m, int site);  // 1-based
+void replace_site(qh_mps* m, int site, cplx* buf);
+void replace_lam(qh_mps* m, int bond, double* buf);
+
+// One SVD of a logical matrix M (mr × mc) held in the caller's buffer x (as M
+// if mr >= mc, else as M^H), with the factor outputs the caller needs.
+struct SvdOut {
+    int factor = 0;          // 0 = U_M, 1 = V_M
+    int conj_transpose = 0;  // write the factor's conjugate transpose
+    const double* rowscale = nullptr;
+    int rowscale_mod = 1, rowscale_div = 1;
+    cplx* dst = nullptr;
+};
+struct SvdTask {
+    cplx* x = nullptr;   // M (mr × mc) if mr >= mc, else M^H (mc × mr); destroyed
+    int mr = 0, mc = 0;
+    int truncate = 0;
+    double threshold = 1e-16;
+    int maxdim = -1;
+    int renormalize = 0;
+    std::vector<SvdOut> outs;
+    double* lam_dst = nullptr;     // kept singular values (may be null)
+    double* sigma_dst = nullptr;   // all sorted singular values (k), may be null
+    int kept = 0;                  // result
+    double discarded = 0.0;        // result
+    int kmax() const;
+};
+// Runs the batch of SVD tasks (Jacobi + select + gathers) and synchronises.
+void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks);
+
+}  // namespace qh

@@ -1,0 +1,158 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances: plain FP64 path, 1e-10 absolute on gauge-invariant quantities
+(state vectors of small chains, ⟨Z_i⟩, overlaps, Λ spectra) and identical
+kept bond dimensions.
+"""
+import numpy as np
+import pytest
+
+from oracle import chain as oc
+from oracle import statevector as sv
+from oracle import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _random_chain(rng, n, d, chi, canon=True):
+    arrays = wl.random_mps_arrays(rng, n, d, wl.max_profile(n, d, chi))
+    if canon:
+        arrays = wl.left_canonize_normalize(arrays)
+    return arrays
+
+
+def _state(psi_gpu):
+    return oc.statevector(oc.Chain(psi_gpu.internal_arrays(), _lams(psi_gpu)))
+
+
+def _lams(psi_gpu):
+    return psi_gpu.lambdas()
+
+
+def test_roundtrip_upload_download(qh):
+    rng = np.random.default_rng(1)
+    arrays = _random_chain(rng, 5, 2, 4, canon=False)
+    psi = qh.from_internal(arrays)
+    got = psi.internal_arrays()
+    for a, b in zip(arrays, got):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b)
+    assert psi.bond_dims() == [2, 4, 4, 2]
+
+
+@pytest.mark.parametrize("n,chi,seed", [(4, 2, 0), (6, 4, 1), (8, 8, 2)])
+def test_evolve2_no_truncation_matches_statevector(qh, n, chi, seed):
+    rng = np.random.default_rng(seed)
+    arrays = _random_chain(rng, n, 2, chi)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    v = oc.statevector(ref)
+    for layer in range(3):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            qh.evolve_(psi, qh.gate(g, b, b + 1))
+            oc.evolve(ref, oc.gate2(g, b))
+            v = sv.apply2(v, g, b)
+    assert psi.bond_dims() == ref.bond_dims()
+    assert np.max(np.abs(_state(psi) - v)) < TOL
+    for b, (lg, lr) in enumerate(zip(psi.lambdas(), ref.lam)):
+        assert (lg is None) == (lr is None)
+        if lg is not None:
+            assert np.max(np.abs(lg - lr)) < TOL
+
+
+def test_config1_parity(qh):
+    arrays, schedule, tr = wl.config1()
+    psi = qh.from_internal(arrays)
+    psi0 = psi.copy()
+    ref = oc.Chain([a.copy() for a in arrays])
+    ref0 = ref.copy()
+    for b, g in schedule:
+        info = {}
+        qh.evolve_(psi, qh.gate(g, b, b + 1), maxdim=tr["maxdim"], info=info)
+        oc.evolve(ref, oc.gate2(g, b), maxdim=tr["maxdim"])
+        assert info["kept"] == ref.bond_dims()[b - 1]
+    assert psi.bond_dims() == ref.bond_dims()
+    n = len(arrays)
+    z_gpu = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in range(1, n + 1)])
+    z_ref = np.array([oc.expect(ref, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)])
+    assert np.max(np.abs(z_gpu - z_ref)) < TOL
+    ov_gpu = qh.overlap(psi, psi0)
+    ov_ref = oc.overlap(ref, ref0)
+    assert abs(ov_gpu - ov_ref) < TOL
+    for i in (1, 7, 16):
+        assert abs(qh.expect(psi, [qh.gate(wl.Z, i)]) - z_ref[i - 1]) < TOL
+
+
+def test_vidal_mode_matches_oracle(qh):
+    rng = np.random.default_rng(11)
+    n = 8
+    arrays = _random_chain(rng, n, 2, 8)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    qh.canonize_(psi)
+    oc.canonize(ref)
+    for layer in range(4):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            qh.simple_update_(psi, qh.gate(g, b, b + 1), maxdim=6)
+            oc.simple_update(ref, oc.gate2(g, b), maxdim=6)
+    assert psi.bond_dims() == ref.bond_dims()
+    assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-9
+    for lg, lr in zip(psi.lambdas(), ref.lam):
+        assert np.max(np.abs(lg - lr)) < 1e-9
+
+
+def test_canonize_invariants(qh):
+    """`Chain_test.jl:308-356`: 2n-1 tensors, value/norm invariant, Σ|Λ|² = ‖ψ‖², canonicality."""
+    rng = np.random.default_rng(3)
+    arrays = [rng.random((4, 4)), rng.random((4, 4, 4)), rng.random((4, 4, 4)), rng.random((4, 4, 4)), rng.random((4, 4))]
+    psi = qh.MPS(arrays)
+    ref = oc.MPS(arrays)
+    can = qh.canonize(psi)
+    assert can.ntensors() == 9
+    v0 = oc.statevector(ref)
+    assert np.max(np.abs(_state(can) - v0)) < 1e-10 * max(1, np.max(np.abs(v0)))
+    nrm = qh.norm(can)
+    assert abs(nrm - qh.norm(psi)) < 1e-10 * nrm
+    for lam in can.lambdas():
+        assert abs(np.sum(lam**2) - nrm**2) < 1e-10 * nrm**2
+    ref_can = oc.canonize(ref.copy())
+    for lg, lr in zip(can.lambdas(), ref_can.lam):
+        assert np.max(np.abs(lg - lr)) < 1e-10 * nrm
+    for i in range(1, 6):
+        c = qh.canonize(psi)
+        if i == 1:
+            assert qh.isleftcanonical(c, i)
+        elif i == 5:
+            qh.contract_between_(c, i - 1, i, direction="right")
+            arrs = c.internal_arrays()
+            arrs[i - 1] = arrs[i - 1] / nrm
+            assert oc.isleftcanonical(oc.Chain(arrs), i)
+        else:
+            qh.contract_between_(c, i - 1, i, direction="right")
+            assert qh.isleftcanonical(c, i)
+
+
+def test_errors(qh):
+    rng = np.random.default_rng(5)
+    arrays = [rng.random((2, 2)), rng.random((2, 2, 2)), rng.random((2, 2))]
+    psi = qh.MPS(arrays)
+    qh.canonize_site_(psi, 2, direction="right", method="svd")
+    with pytest.raises(qh.MissingSchmidtCoefficientsException):
+        qh.truncate_(psi, (1, 2), maxdim=1)
+    t = qh.truncate(psi, (2, 3), maxdim=1)
+    assert t.bond_dims()[1] == 1
+    lam = psi.lambda_((2, 3))
+    t = qh.truncate(psi, (2, 3), threshold=lam[1] + 0.1)
+    assert t.bond_dims()[1] == 1
+    with pytest.raises(qh.ArgumentError):
+        qh.canonize_site_(psi, 1, direction="left")
+    with pytest.raises(qh.ArgumentError):
+        qh.canonize_site_(psi, 3, direction="right")
+    with pytest.raises(qh.ArgumentError):
+        qh.evolve_(psi, qh.gate(np.eye(4), 1, 3))
+    with pytest.raises(qh.ArgumentError):
+        qh.normalize_(qh.MPS(arrays), 1)
