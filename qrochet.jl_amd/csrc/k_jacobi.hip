@@ -24,7 +24,8 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
 
 template <int BC>
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
-                                                      const int2* __restrict__ tiles, int round, double tol) {
+                                                      const int2* __restrict__ tiles, int round, double tol,
+                                                      double floor_rel2, int sort, int debug) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     __shared__ double sRot[NP][6];
     __shared__ int sPair[NP][2];
     __shared__ int sCol[BC];
+    __shared__ int sPerm[BC];
     __shared__ int sFlag;
 
     const int tid = threadIdx.x;
@@ -53,6 +55,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     __syncthreads();
 
     const int p = it.p;
+    // columns at or below the rounding floor are frozen (treated as zero)
+    const double gfloor = floor_rel2 * (*it.fro2);
     // ---------------- Gram matrix ----------------
     const int lane = tid & 63, wv = tid >> 6;
     const int ti = lane & 7, tj = lane >> 3;
@@ -97,6 +101,14 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         __syncthreads();
     }
 
+    // make G exactly Hermitian so that every test below sees the same value for (i, j)
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int i = e / BC, j = e % BC;
+        if (i > j) sG[i][j] = cconj(sG[j][i]);
+        else if (i == j) sG[i][i].y = 0.0;
+    }
+    __syncthreads();
+
     // ---------------- convergence test ----------------
     if (tid == 0) sFlag = 0;
     __syncthreads();
@@ -105,12 +117,16 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         if (i < j) {
             const double gii = sG[i][i].x, gjj = sG[j][j].x;
             const double2 g = sG[i][j];
-            if (gii > 0.0 && gjj > 0.0 && sqrt(cabs2(g)) > tol * sqrt(gii) * sqrt(gjj)) sFlag = 1;
+            if (gii > gfloor && gjj > gfloor && sqrt(cabs2(g)) > tol * sqrt(gii) * sqrt(gjj)) {
+                sFlag = 1;
+                if (debug)
+                    printf("[jr] round %d pair(%d,%d) cols %d %d cos/tol %.6f gii %.17e gjj %.17e\n", round, bi, bj,
+                           sCol[i], sCol[j], sqrt(cabs2(g)) / (sqrt(gii) * sqrt(gjj)) / tol, gii, gjj);
+            }
         }
     }
     __syncthreads();
     if (!sFlag) return;
-    if (tid == 0) atomicAdd(it.rot, 1);
 
     // ---------------- EVD of G by cyclic Jacobi ----------------
     for (int e = tid; e < BC * BC; e += 256) {
@@ -131,7 +147,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
                 const double2 gab = sG[a][b];
                 const double mag = sqrt(cabs2(gab));
                 double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = gaa, nbb = gbb;
-                if (gaa > 0.0 && gbb > 0.0 && mag > tol * sqrt(gaa) * sqrt(gbb)) {
+                if (gaa > gfloor && gbb > gfloor && mag > tol * sqrt(gaa) * sqrt(gbb)) {
                     const double zeta = (gbb - gaa) / (2.0 * mag);
                     const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     c = 1.0 / sqrt(1.0 + t * t);
@@ -189,6 +205,45 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         const int any = sFlag;
         __syncthreads();
         if (!any) break;
+        if (sweep == 0 && tid == 0) atomicAdd(it.rot, 1);  // counted only if a rotation happened
+    }
+
+    // ---------------- order eigenpairs: larger eigenvalues to the lower block ----------------
+    // (de Rijk-style sorting; valid columns only, invalid positions keep e_j).  Used in the
+    // first sweeps only: afterwards the plain cyclic ordering finishes with its usual
+    // quadratic convergence (sorting in a parallel ordering can lock into a cycle).
+    if (sort) {
+    if (tid == 0) {
+        int slot[BC], src[BC];
+        int nv = 0;
+        const int first = (bi < bj) ? 0 : B;
+        for (int h = 0; h < 2; ++h) {
+            const int base = (h == 0) ? first : B - first;
+            for (int j = 0; j < B; ++j)
+                if (sCol[base + j] >= 0) slot[nv++] = base + j;
+        }
+        for (int k = 0; k < nv; ++k) src[k] = slot[k];
+        for (int k = 1; k < nv; ++k) {  // insertion sort by eigenvalue, descending (stable)
+            const int cur = src[k];
+            const double ev = sG[cur][cur].x;
+            int m = k - 1;
+            while (m >= 0 && sG[src[m]][src[m]].x < ev) { src[m + 1] = src[m]; --m; }
+            src[m + 1] = cur;
+        }
+        for (int j = 0; j < BC; ++j) sPerm[j] = j;
+        for (int k = 0; k < nv; ++k) sPerm[slot[k]] = src[k];
+    }
+    __syncthreads();
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int i = e / BC, j = e % BC;
+        sX[i][j] = sQ[i][sPerm[j]];
+    }
+    __syncthreads();
+    for (int e = tid; e < BC * BC; e += 256) {
+        const int i = e / BC, j = e % BC;
+        sQ[i][j] = sX[i][j];
+    }
+    __syncthreads();
     }
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
@@ -226,12 +281,33 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
 }
 
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, hipStream_t s) {
+                         double tol, double floor_rel2, int sort, int debug, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
-        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol);
+        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug);
     else
-        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol);
+        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug);
+}
+
+// ||X||_F^2 of every item: each workgroup reduces one chunk and adds it atomically
+__global__ __launch_bounds__(256) void k_frob2(const JacItem* __restrict__ items, const int2* __restrict__ tiles) {
+    __shared__ double sRed[4];
+    const int2 tt = tiles[blockIdx.x];
+    const JacItem it = items[tt.x];
+    const size_t total = (size_t)it.p * it.q;
+    const size_t beg = (size_t)tt.y * FROB_CHUNK;
+    const size_t end = beg + FROB_CHUNK < total ? beg + FROB_CHUNK : total;
+    double s = 0.0;
+    for (size_t e = beg + threadIdx.x; e < end; e += 256) s += cabs2(it.x[e]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sRed[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(it.fro2, sRed[0] + sRed[1] + sRed[2] + sRed[3]);
+}
+void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_frob2<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 
 // V <- I for every item (grid: items × column tiles)

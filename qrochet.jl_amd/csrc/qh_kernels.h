@@ -66,10 +66,16 @@ struct JacItem {
     int nb;         // column blocks (even >= 2)
     int* rot;       // rotations performed in the current window (device counter)
     int* conv;      // 1 once converged (kernels skip the item)
+    double* fro2;   // ||X||_F^2 (device scalar, filled before the sweeps)
 };
+// Columns with ||x||^2 <= JAC_FLOOR(p)^2 ||X||_F^2 are below the rounding floor of the
+// rotations and are treated as zero (frozen): JAC_FLOOR(p) = 4 sqrt(p) eps.
+// ||X||_F^2 per item (tile map: item, chunk of 64k elements)
+void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, hipStream_t s);
+                         double tol, double floor_rel2, int sort, int debug, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // converged <- (no rotation in the window); *remaining = items still rotating

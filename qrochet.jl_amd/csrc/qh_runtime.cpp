@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace qh {
@@ -186,6 +188,9 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     int rmax = 1;
     int* counters = dalloc<int>(c, 2 * items.size() + 1);
     QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 1) * sizeof(int), c->stream));
+    double* fro2 = dalloc<double>(c, items.size());
+    QH_HIP(hipMemsetAsync(fro2, 0, items.size() * sizeof(double), c->stream));
+    std::vector<int2> frob_tiles;
     for (size_t i = 0; i < items.size(); ++i) {
         JacItem& it = items[i];
         int nb = (it.q + block - 1) / block;
@@ -194,6 +199,9 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
         it.nb = nb;
         it.rot = counters + 2 * i;
         it.conv = counters + 2 * i + 1;
+        it.fro2 = fro2 + i;
+        const size_t tot = (size_t)it.p * it.q;
+        for (size_t k = 0; k * FROB_CHUNK < tot; ++k) frob_tiles.push_back(make_int2((int)i, (int)k));
         rmax = std::max(rmax, nb - 1);
         for (int k = 0; k < it.q; ++k) id_tiles.push_back(make_int2((int)i, k));
         for (int k = 0; k < nb / 2; ++k) tiles.push_back(make_int2((int)i, k));
@@ -202,13 +210,17 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     JacItem* di = upload(c, items);
     int2* dti = upload(c, id_tiles);
     int2* dt = upload(c, tiles);
+    int2* dtf = upload(c, frob_tiles);
     launch_set_identity_items(di, dti, (int)id_tiles.size(), c->stream);
+    launch_frob2(di, dtf, (int)frob_tiles.size(), c->stream);
     // Relative orthogonality target |x_i^H x_j| <= tol ||x_i|| ||x_j||.  The Gram
     // entries carry O(sqrt(p)·ε) rounding noise, so the target sits a factor 16
     // above it; otherwise noise-triggered rotations never stop.
     int pmax = 1;
     for (const JacItem& it : items) pmax = std::max(pmax, it.p);
     const double tol = 16.0 * std::sqrt((double)std::max(pmax, 4)) * DBL_EPSILON;
+    const double floor_rel = 4.0 * std::sqrt((double)std::max(pmax, 4)) * DBL_EPSILON;
+    const double floor_rel2 = floor_rel * floor_rel;
     // algorithmic work per checked / rotated block pair (for the timing report)
     std::vector<double> gram_f(items.size()), upd_f(items.size()), gram_b(items.size()), upd_b(items.size());
     for (size_t i = 0; i < items.size(); ++i) {
@@ -226,7 +238,18 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
         for (int r = 0; r < rmax; ++r) {
             const int g = sweep * rmax + r;
             Timed tm(c, "jacobi_round");
-            launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, c->stream);
+            static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
+            launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, floor_rel2, sweep < c->jac_sort_sweeps ? 1 : 0,
+                                kdbg, c->stream);
+        }
+        static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
+        if (dbg) {
+            readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
+            std::fprintf(stderr, "[jacobi] sweep %d rmax %d rot:", sweep, rmax);
+            for (size_t i = 0; i < items.size(); ++i)
+                std::fprintf(stderr, " %d/%d(%dx%d,nb%d)", hrot[2 * i], hrot[2 * i + 1], items[i].p, items[i].q,
+                             items[i].nb);
+            std::fprintf(stderr, "\n");
         }
         if (c->timing) {
             readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
@@ -248,8 +271,37 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
             break;
         }
     }
+    if (!done) {
+        std::vector<int> h(2 * items.size());
+        readback(c, h.data(), counters, h.size() * sizeof(int));
+        std::string msg = "Jacobi SVD did not converge within " + std::to_string(c->jac_max_sweeps) + " sweeps:";
+        if (const char* dump = std::getenv("QH_DEBUG_JACOBI_DUMP")) {
+            for (size_t i = 0; i < items.size(); ++i)
+                if (!h[2 * i + 1]) {
+                    std::vector<cplx> hx((size_t)items[i].p * items[i].q);
+                    readback(c, hx.data(), items[i].x, hx.size() * sizeof(cplx));
+                    if (FILE* f = std::fopen(dump, "wb")) {
+                        int dims[2] = {items[i].p, items[i].q};
+                        std::fwrite(dims, sizeof(int), 2, f);
+                        std::fwrite(hx.data(), sizeof(cplx), hx.size(), f);
+                        std::fclose(f);
+                    }
+                    break;
+                }
+        }
+        int shown = 0;
+        for (size_t i = 0; i < items.size() && shown < 4; ++i)
+            if (!h[2 * i + 1]) {
+                msg += " item " + std::to_string(i) + " (" + std::to_string(items[i].p) + "x" +
+                       std::to_string(items[i].q) + ")";
+                ++shown;
+            }
+        dfree(c, counters);
+        dfree(c, fro2);
+        throw Error(QH_ERR_NOT_CONVERGED, msg);
+    }
     dfree(c, counters);
-    if (!done) throw Error(QH_ERR_NOT_CONVERGED, "Jacobi SVD did not converge within the sweep limit");
+    dfree(c, fro2);
 }
 
 // ---------------------------------------------------------------------------

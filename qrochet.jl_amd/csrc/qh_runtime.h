@@ -54,6 +54,7 @@ struct qh_context {
     // Jacobi parameters
     int jac_block = 16;
     int jac_max_sweeps = 40;
+    int jac_sort_sweeps = 2;  // sweeps that order eigenpairs by size (then plain cyclic)
 };
 
 struct qh_mps {

@@ -156,3 +156,45 @@ def test_errors(qh):
         qh.evolve_(psi, qh.gate(np.eye(4), 1, 3))
     with pytest.raises(qh.ArgumentError):
         qh.normalize_(qh.MPS(arrays), 1)
+
+
+def test_batched_layer_equals_single_calls(qh):
+    """A brickwork layer through qh_evolve2_batch gives the per-call result."""
+    arrays, schedule, tr = wl.config1()
+    a = qh.from_internal(arrays)
+    b = qh.from_internal(arrays)
+    n = len(arrays)
+    pos = 0
+    for layer in range(8):
+        bonds = wl.brickwork_bonds(n, layer)
+        ups = schedule[pos: pos + len(bonds)]
+        pos += len(bonds)
+        qh.evolve_layer_([(a, qh.gate(g, bb, bb + 1)) for bb, g in ups], maxdim=tr["maxdim"])
+        for bb, g in ups:
+            qh.evolve_(b, qh.gate(g, bb, bb + 1), maxdim=tr["maxdim"])
+        assert a.bond_dims() == b.bond_dims()
+    za = qh.expect_many(a, [[qh.gate(wl.Z, i)] for i in range(1, n + 1)])
+    zb = qh.expect_many(b, [[qh.gate(wl.Z, i)] for i in range(1, n + 1)])
+    assert np.max(np.abs(za - zb)) < 1e-12
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_ising_trotter_prefix_parity(qh, batched):
+    """Workload 2 (n=64, cutoff 1e-12, maxdim 256) for its first 2 Trotter steps."""
+    arrays, layers, tr, _ = wl.config2(steps=2)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    for li, layer in enumerate(layers):
+        if batched:
+            qh.evolve_layer_([(psi, qh.gate(g, b, b + 1)) for b, g in layer], **tr)
+        else:
+            for b, g in layer:
+                qh.evolve_(psi, qh.gate(g, b, b + 1), **tr)
+        for b, g in layer:
+            oc.evolve(ref, oc.gate2(g, b), **tr)
+        assert psi.bond_dims() == ref.bond_dims(), li
+    n = 64
+    zs = [[qh.gate(wl.Z, i)] for i in range(1, n + 1)]
+    z_gpu = qh.expect_many(psi, zs)
+    z_ref = np.array([oc.expect(ref, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)])
+    assert np.max(np.abs(z_gpu - z_ref)) < TOL
