@@ -90,7 +90,7 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     ctx.set_timing(False)
-    classes = ["jacobi_round", "theta", "select", "gather", "gemm", "jacobi_sweep"]
+    classes = ["jacobi_round", "theta", "qr", "qr_apply", "select", "gather", "gemm", "jacobi_sweep"]
     timers = {k: ctx.timing(k) for k in classes}
     elapsed = t1 - t0
     value = n_updates * args.steps / elapsed

@@ -25,7 +25,7 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
 template <int BC>
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
                                                       const int2* __restrict__ tiles, int round, double tol,
-                                                      double floor_rel2, int sort, int debug) {
+                                                      double floor_rel2, int sort, int debug, int max_inner) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
@@ -135,7 +135,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     }
     __syncthreads();
 #pragma unroll 1
-    for (int sweep = 0; sweep < 12; ++sweep) {
+    // one block pair per matrix: solve the block exactly; otherwise an approximate
+    // diagonaliser suffices (the outer sweeps re-check every pair on fresh Grams)
+    const int inner_cap = (it.nb == 2) ? 12 : max_inner;
+    for (int sweep = 0; sweep < inner_cap; ++sweep) {
         if (tid == 0) sFlag = 0;
         __syncthreads();
 #pragma unroll 1
@@ -281,12 +284,12 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
 }
 
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, double floor_rel2, int sort, int debug, hipStream_t s) {
+                         double tol, double floor_rel2, int sort, int debug, int max_inner, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
-        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug);
+        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug, max_inner);
     else
-        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug);
+        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug, max_inner);
 }
 
 // ||X||_F^2 of every item: each workgroup reduces one chunk and adds it atomically
@@ -463,7 +466,7 @@ __global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ i
         const int j = j0 + jj, c = c0 + cc;
         double2 v = cz();
         if (j < kept && c < g.rows) {
-            const int src_col = g.perm[j];
+            const int src_col = g.perm ? g.perm[j] : j;
             v = g.src[c + (size_t)g.ld_src * src_col];
             double f = 1.0;
             if (g.divide_sigma) {
@@ -474,7 +477,8 @@ __global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ i
             v = cscale(v, f);
         }
         if (!g.conj_transpose) {
-            if (j < kept && c < g.rows) g.out[c + (size_t)g.rows * j] = v;
+            const size_t ldo = g.ld_out > 0 ? g.ld_out : g.rows;
+            if (j < kept && c < g.rows) g.out[c + ldo * j] = v;
         } else {
             sT[jj][cc] = cconj(v);
         }

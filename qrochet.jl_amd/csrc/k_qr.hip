@@ -1,0 +1,238 @@
+// Batched blocked Householder QR (complex FP64), used to precondition the
+// one-sided Jacobi SVD:  X = Q1 R1,  R1^H = Q2 R2,  Jacobi on R2^H.  The two
+// triangular factors grade the columns so that the Jacobi sweeps converge in a
+// handful of sweeps (the preconditioning of Drmac & Veselic, "New fast and
+// accurate Jacobi SVD algorithm", SIAM J. Matrix Anal. Appl. 29 (2008)).
+//
+// Panels of QR_NB columns: one workgroup factors a panel (LAPACK zlarfg
+// reflectors, H^H x = beta e1 with H = I - tau v v^H, real beta) and builds the
+// compact-WY triangle T (Q_panel = I - V T V^H); a second kernel applies
+// Q_panel^H (factorisation) or Q_panel (forming Q·M) to column tiles.
+#include "qh_kernels.h"
+#include "qh_complex.h"
+
+namespace qh {
+
+constexpr int QR_TC = 16;  // target columns per apply workgroup
+constexpr int QR_CH = 32;  // rows per LDS chunk
+
+// block-wide sum of K complex values per thread (256 threads); result in out[K]
+template <int K>
+__device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K */, double2* out /* K */) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double x = v[k].x, y = v[k].y;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_xor(x, off, 64);
+            y += __shfl_xor(y, off, 64);
+        }
+        if (lane == 0) sRed[wv * K + k] = make_double2(x, y);
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double2 s = cz();
+        for (int w = 0; w < 4; ++w) s = cadd(s, sRed[w * K + threadIdx.x]);
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ items, int j0) {
+    __shared__ double2 sRed[4 * QR_NB];
+    __shared__ double2 sDot[QR_NB];
+    __shared__ double2 sT[QR_NB][QR_NB + 1];
+    __shared__ double2 sScale;
+    __shared__ double sBeta;
+    __shared__ double2 sTau[QR_NB];
+    const QrItem it = items[blockIdx.x];
+    if (j0 >= it.n) return;
+    const int tid = threadIdx.x;
+    const int m = it.m;
+    const int nb = min(QR_NB, it.n - j0);
+    double2* a = it.a;
+    double2* v = it.v;
+    for (int c = 0; c < nb; ++c) {
+        const int col = j0 + c, row0 = j0 + c;
+        // ---- reflector for a[row0:, col]
+        double2 part[1] = {cz()};
+        for (int r = row0 + 1 + tid; r < m; r += 256) part[0].x += cabs2(a[r + (size_t)it.lda * col]);
+        block_sum<1>(part, sRed, sDot);
+        if (tid == 0) {
+            const double xnorm2 = sDot[0].x;
+            const double2 alpha = a[row0 + (size_t)it.lda * col];
+            double2 tau = cz(), scale = cz();
+            double beta = alpha.x;
+            if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+                const double nrm = sqrt(cabs2(alpha) + xnorm2);
+                beta = alpha.x >= 0.0 ? -nrm : nrm;
+                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                // scale = 1 / (alpha - beta)
+                const double2 den = make_double2(alpha.x - beta, alpha.y);
+                const double dd = cabs2(den);
+                scale = make_double2(den.x / dd, -den.y / dd);
+            }
+            sTau[c] = tau;
+            sScale = scale;
+            sBeta = beta;
+            a[row0 + (size_t)it.lda * col] = make_double2(beta, 0.0);
+        }
+        __syncthreads();
+        const double2 scale = sScale;
+        for (int r = j0 + tid; r < m; r += 256) {
+            double2 vr;
+            if (r < row0) vr = cz();
+            else if (r == row0) vr = make_double2(1.0, 0.0);
+            else {
+                vr = cmul(a[r + (size_t)it.lda * col], scale);
+                a[r + (size_t)it.lda * col] = cz();
+            }
+            v[r + (size_t)it.m * col] = vr;
+        }
+        __syncthreads();
+        // ---- dots: w_k = v^H a_k (k > c), s_i = v_i^H v (i < c)
+        double2 dots[QR_NB];
+#pragma unroll
+        for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
+        for (int r = row0 + tid; r < m; r += 256) {
+            const double2 vr = v[r + (size_t)it.m * col];
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) {
+                if (k > c && k < nb) dots[k] = cfmac(vr, a[r + (size_t)it.lda * (j0 + k)], dots[k]);
+                else if (k < c) dots[k] = cfmac(v[r + (size_t)it.m * (j0 + k)], vr, dots[k]);
+            }
+        }
+        block_sum<QR_NB>(dots, sRed, sDot);
+        // ---- T column c:  T[i][c] = -tau_c Σ_{l=i}^{c-1} T[i][l] s_l ;  T[c][c] = tau_c
+        if (tid < QR_NB) {
+            const int i = tid;
+            double2 val = cz();
+            if (i < c) {
+                double2 acc = cz();
+                for (int l = i; l < c; ++l) acc = cfma(sT[i][l], sDot[l], acc);
+                const double2 tc = sTau[c];
+                val = make_double2(-(tc.x * acc.x - tc.y * acc.y), -(tc.x * acc.y + tc.y * acc.x));
+            } else if (i == c) {
+                val = sTau[c];
+            }
+            sT[i][c] = val;
+        }
+        // ---- update remaining panel columns: a_k -= conj(tau) w_k v
+        const double2 ctau = cconj(sTau[c]);
+        for (int r = row0 + tid; r < m; r += 256) {
+            const double2 vr = v[r + (size_t)it.m * col];
+            for (int k = c + 1; k < nb; ++k) {
+                const double2 f = cmul(ctau, sDot[k]);
+                double2& y = a[r + (size_t)it.lda * (j0 + k)];
+                y = csub(y, cmul(f, vr));
+            }
+        }
+        __syncthreads();
+    }
+    // store T (nb × nb, column-major, ld QR_NB)
+    double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
+    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+        const int i = e % QR_NB, j = e / QR_NB;
+        t[i + QR_NB * j] = (i < nb && j < nb) ? sT[i][j] : cz();
+    }
+}
+
+// Y[j0:, cols] <- (I - V T' V^H) Y[j0:, cols],  T' = T^H (adjoint) or T
+__global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
+                                                  int j0, int adjoint) {
+    __shared__ double2 sV[QR_CH][QR_NB + 1];
+    __shared__ double2 sY[QR_CH][QR_TC + 1];
+    __shared__ double2 sW[QR_NB][QR_TC + 1];
+    __shared__ double2 sT[QR_NB][QR_NB + 1];
+    const int2 tt = tiles[blockIdx.x];
+    const QrApplyItem it = items[tt.x];
+    const int nb = min(QR_NB, it.nrefl - j0);
+    if (nb <= 0) return;
+    const int c0 = (it.trailing ? j0 + nb : 0) + tt.y * QR_TC;
+    const int tc = min(QR_TC, it.ncols - c0);
+    if (tc <= 0) return;
+    const int tid = threadIdx.x;
+    const int m = it.m;
+    const double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
+    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+        const int i = e % QR_NB, j = e / QR_NB;
+        sT[i][j] = t[i + QR_NB * j];
+    }
+    // phase 1: W = V^H Y   (thread: a = tid / 16, col = tid % 16)
+    const int wa = tid / QR_TC, wc = tid % QR_TC;
+    double2 w = cz();
+    for (int r0 = j0; r0 < m; r0 += QR_CH) {
+        for (int e = tid; e < QR_CH * QR_NB; e += 256) {
+            const int rr = e % QR_CH, k = e / QR_CH;
+            const int r = r0 + rr;
+            sV[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)it.m * (j0 + k)] : cz();
+        }
+        for (int e = tid; e < QR_CH * QR_TC; e += 256) {
+            const int rr = e % QR_CH, k = e / QR_CH;
+            const int r = r0 + rr;
+            sY[rr][k] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int rr = 0; rr < QR_CH; ++rr) w = cfmac(sV[rr][wa], sY[rr][wc], w);
+        __syncthreads();
+    }
+    sW[wa][wc] = w;
+    __syncthreads();
+    // phase 2: W2 = T' W
+    double2 w2 = cz();
+    for (int b = 0; b < QR_NB; ++b) {
+        const double2 tab = adjoint ? cconj(sT[b][wa]) : sT[wa][b];
+        w2 = cfma(tab, sW[b][wc], w2);
+    }
+    __syncthreads();
+    sW[wa][wc] = w2;
+    __syncthreads();
+    // phase 3: Y -= V W2   (thread: row rr = tid % 32, columns (tid / 32) * 2 + {0,1})
+    const int rr = tid % QR_CH, cg = (tid / QR_CH) * 2;
+    for (int r0 = j0; r0 < m; r0 += QR_CH) {
+        const int r = r0 + rr;
+        if (r < m) {
+            double2 acc0 = cz(), acc1 = cz();
+            for (int k = 0; k < nb; ++k) {
+                const double2 vr = it.v[r + (size_t)it.m * (j0 + k)];
+                acc0 = cfma(vr, sW[k][cg], acc0);
+                acc1 = cfma(vr, sW[k][cg + 1], acc1);
+            }
+            if (cg < tc) {
+                double2& y = it.y[r + (size_t)it.ldy * (c0 + cg)];
+                y = csub(y, acc0);
+            }
+            if (cg + 1 < tc) {
+                double2& y = it.y[r + (size_t)it.ldy * (c0 + cg + 1)];
+                y = csub(y, acc1);
+            }
+        }
+    }
+}
+
+void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
+    if (count <= 0) return;
+    k_qr_panel<<<count, 256, 0, s>>>(d_items, j0);
+}
+void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_qr_apply<<<ntiles, 256, 0, s>>>(d_items, d_tiles, j0, adjoint);
+}
+int qr_tile_cols() { return QR_TC; }
+
+// dst (n × n) = upper triangle of src (ld lds)^H, zeros above the diagonal of dst
+__global__ void k_upper_h(const double2* __restrict__ src, int lds, int n, double2* __restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);  // dst(i, j)
+    dst[e] = (i >= j) ? cconj(src[j + (size_t)lds * i]) : cz();
+}
+void launch_upper_h(const cplx* src, int lds, int n, cplx* dst, hipStream_t s) {
+    const size_t t = (size_t)n * n;
+    if (t == 0) return;
+    k_upper_h<<<(int)((t + 255) / 256), 256, 0, s>>>(src, lds, n, dst);
+}
+
+}  // namespace qh

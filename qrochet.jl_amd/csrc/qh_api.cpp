@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -637,6 +638,8 @@ int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
         if (block != 0 && block != 8 && block != 16) throw Error(QH_ERR_ARGUMENT, "Jacobi block must be 8 or 16");
         if (block) c->jac_block = block;
         if (max_sweeps > 0) c->jac_max_sweeps = max_sweeps;
+        if (const char* e = std::getenv("QH_JACOBI_MAX_INNER")) c->jac_max_inner = std::max(1, std::atoi(e));
+        if (const char* e = std::getenv("QH_JACOBI_PRECONDITION")) c->jac_precondition = std::atoi(e);
     });
 }
 

@@ -75,7 +75,7 @@ void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipSt
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, double floor_rel2, int sort, int debug, hipStream_t s);
+                         double tol, double floor_rel2, int sort, int debug, int max_inner, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // converged <- (no rotation in the window); *remaining = items still rotating
@@ -117,9 +117,39 @@ struct GatherItem {
     cplx* out;
     int conj_transpose;
     int max_cols;
+    int ld_out;     // non-transposed output leading dimension (0: rows)
 };
 // tile map: (item, tile) over 32 × 32 tiles, tile = jt * ceil(rows/32) + ct
 void launch_gather_items(const GatherItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// batched blocked Householder QR (preconditioner of the Jacobi SVD)
+// ---------------------------------------------------------------------------
+constexpr int QR_NB = 16;  // panel width
+struct QrItem {
+    cplx* a;        // m × n (m >= n), factored in place: R in the upper triangle
+    int m, n, lda;
+    cplx* v;        // explicit reflectors (m × n, ld m): zero above the diagonal, 1 on it
+    cplx* t;        // compact-WY triangles, panel k at t + k·QR_NB², QR_NB × QR_NB column-major
+};
+struct QrApplyItem {
+    const cplx* v;  // reflectors of a QrItem (ld m)
+    const cplx* t;
+    int m;          // rows of the reflectors / target
+    int nrefl;      // number of reflectors (n of the factorisation)
+    cplx* y;        // target (m × ncols, ld ldy)
+    int ldy;
+    int ncols;
+    int trailing;   // 1: columns from j0 + panel width (factorisation update); 0: all columns
+};
+// panel j0 of every item (grid = items; items with n <= j0 return at once)
+void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s);
+// apply panel j0's block reflector: adjoint = 1 -> Q_panel^H (factorisation), 0 -> Q_panel
+// tile map: (item, column tile of qr_tile_cols() columns)
+void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s);
+int qr_tile_cols();
+// dst (n × n) = (upper triangle of src)^H, zero above the diagonal
+void launch_upper_h(const cplx* src, int lds, int n, cplx* dst, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // small kernels

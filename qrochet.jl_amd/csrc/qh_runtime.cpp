@@ -240,7 +240,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
             Timed tm(c, "jacobi_round");
             static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
             launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, floor_rel2, sweep < c->jac_sort_sweeps ? 1 : 0,
-                                kdbg, c->stream);
+                                kdbg, c->jac_max_inner, c->stream);
         }
         static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
         if (dbg) {
@@ -336,6 +336,57 @@ void run_gather(qh_context* c, std::vector<GatherItem>& items) {
     launch_gather_items(di, dt, (int)tiles.size(), c->stream);
 }
 
+
+// ---------------------------------------------------------------------------
+// Householder QR (preconditioner)
+// ---------------------------------------------------------------------------
+void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
+    if (jobs.empty()) return;
+    std::vector<QrItem> items(jobs.size());
+    std::vector<QrApplyItem> app(jobs.size());
+    std::vector<int2> tiles;
+    int nmax = 0;
+    const int tc = qr_tile_cols();
+    double flops = 0;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const QrJob& j = jobs[i];
+        items[i] = QrItem{j.a, j.m, j.n, j.lda, j.v, j.t};
+        app[i] = QrApplyItem{j.v, j.t, j.m, j.n, j.a, j.lda, j.n, 1};
+        nmax = std::max(nmax, j.n);
+        const int ntile = (std::max(j.n - QR_NB, 0) + tc - 1) / tc;
+        for (int k = 0; k < ntile; ++k) tiles.push_back(make_int2((int)i, k));
+        flops += 32.0 / 3.0 * j.n * (double)j.n * (3.0 * j.m - j.n) / 2.0;
+    }
+    QrItem* di = upload(c, items);
+    QrApplyItem* da = upload(c, app);
+    int2* dt = upload(c, tiles);
+    Timed tm(c, "qr", flops, 0.0);
+    for (int j0 = 0; j0 < nmax; j0 += QR_NB) {
+        launch_qr_panel(di, (int)items.size(), j0, c->stream);
+        launch_qr_apply(da, dt, (int)tiles.size(), j0, 1, c->stream);
+    }
+}
+
+void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs) {
+    if (jobs.empty()) return;
+    std::vector<QrApplyItem> app(jobs.size());
+    std::vector<int2> tiles;
+    int nmax = 0;
+    const int tc = qr_tile_cols();
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const QrJob& q = *jobs[i].qr;
+        app[i] = QrApplyItem{q.v, q.t, q.m, q.n, jobs[i].y, q.m, jobs[i].ncols, 0};
+        nmax = std::max(nmax, q.n);
+        for (int k = 0; k < (jobs[i].ncols + tc - 1) / tc; ++k) tiles.push_back(make_int2((int)i, k));
+    }
+    if (tiles.empty()) return;
+    QrApplyItem* da = upload(c, app);
+    int2* dt = upload(c, tiles);
+    Timed tm(c, "qr_apply");
+    const int last = ((nmax - 1) / QR_NB) * QR_NB;
+    for (int j0 = last; j0 >= 0; j0 -= QR_NB) launch_qr_apply(da, dt, (int)tiles.size(), j0, 0, c->stream);
+}
+
 int SvdTask::kmax() const {
     const int k = std::min(mr, mc);
     if (truncate && maxdim >= 0) return std::min(k, maxdim);
@@ -344,44 +395,89 @@ int SvdTask::kmax() const {
 
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     if (tasks.empty()) return;
-    // workspace: V (q×q), sigma (q), perm (q), kept (int), discarded (double)
+    const size_t nt = tasks.size();
+    // workspace layout per task
     size_t bytes = 0;
-    std::vector<size_t> off_v(tasks.size()), off_s(tasks.size()), off_p(tasks.size()), off_l(tasks.size());
     auto take = [&](size_t b) {
         size_t o = bytes;
         bytes += (b + 255) & ~size_t(255);
         return o;
     };
-    for (size_t i = 0; i < tasks.size(); ++i) {
-        const size_t q = std::min(tasks[i].mr, tasks[i].mc);
-        off_v[i] = take(q * q * sizeof(cplx));
-        off_s[i] = take(q * sizeof(double));
-        off_p[i] = take(q * sizeof(int));
-        off_l[i] = take(q * sizeof(double));
+    struct Off {
+        size_t v, s, p, l, v1, t1, y0, v2, t2, yj, ux, vx;
+        bool pre;
+        int p_, q_, k_;
+    };
+    std::vector<Off> off(nt);
+    for (size_t i = 0; i < nt; ++i) {
+        const SvdTask& t = tasks[i];
+        Off& o = off[i];
+        o.p_ = std::max(t.mr, t.mc);
+        o.q_ = std::min(t.mr, t.mc);
+        o.k_ = t.kmax();
+        const size_t p = o.p_, q = o.q_, k = o.k_;
+        o.pre = c->jac_precondition && (int)q > 2 * c->jac_block;
+        o.v = take(q * q * sizeof(cplx));
+        o.s = take(q * sizeof(double));
+        o.p = take(q * sizeof(int));
+        o.l = take(q * sizeof(double));
+        if (o.pre) {
+            const size_t tq = ((q + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
+            o.v1 = take(p * q * sizeof(cplx));
+            o.t1 = take(tq * sizeof(cplx));
+            o.y0 = take(q * q * sizeof(cplx));
+            o.v2 = take(q * q * sizeof(cplx));
+            o.t2 = take(tq * sizeof(cplx));
+            o.yj = take(q * q * sizeof(cplx));
+            o.ux = take(p * std::max<size_t>(k, 1) * sizeof(cplx));
+            o.vx = take(q * std::max<size_t>(k, 1) * sizeof(cplx));
+        }
     }
-    const size_t off_k = take(tasks.size() * sizeof(int));
-    const size_t off_d = take(tasks.size() * sizeof(double));
+    const size_t off_k = take(nt * sizeof(int));
+    const size_t off_d = take(nt * sizeof(double));
     char* ws = static_cast<char*>(dalloc_bytes(c, bytes));
+    auto P = [&](size_t o) { return reinterpret_cast<cplx*>(ws + o); };
     int* d_kept = reinterpret_cast<int*>(ws + off_k);
     double* d_disc = reinterpret_cast<double*>(ws + off_d);
-    std::vector<JacItem> jac(tasks.size());
-    std::vector<SelectItem> sel(tasks.size());
-    for (size_t i = 0; i < tasks.size(); ++i) {
+
+    // ---- QR preconditioning: X = Q1 R1, R1^H = Q2 R2, Jacobi on Y = R2^H
+    std::vector<QrJob> qr1(nt), qr2(nt);
+    std::vector<QrJob> f1, f2;
+    for (size_t i = 0; i < nt; ++i) {
+        if (!off[i].pre) continue;
+        const int p = off[i].p_, q = off[i].q_;
+        qr1[i] = QrJob{tasks[i].x, p, q, p, P(off[i].v1), P(off[i].t1)};
+        qr2[i] = QrJob{P(off[i].y0), q, q, q, P(off[i].v2), P(off[i].t2)};
+        f1.push_back(qr1[i]);
+        f2.push_back(qr2[i]);
+    }
+    if (!f1.empty()) {
+        run_qr_factor(c, f1);
+        for (size_t i = 0; i < nt; ++i)
+            if (off[i].pre) launch_upper_h(tasks[i].x, off[i].p_, off[i].q_, P(off[i].y0), c->stream);
+        run_qr_factor(c, f2);
+        for (size_t i = 0; i < nt; ++i)
+            if (off[i].pre) launch_upper_h(P(off[i].y0), off[i].q_, off[i].q_, P(off[i].yj), c->stream);
+    }
+
+    std::vector<JacItem> jac(nt);
+    std::vector<SelectItem> sel(nt);
+    for (size_t i = 0; i < nt; ++i) {
         const SvdTask& t = tasks[i];
-        const int p = std::max(t.mr, t.mc), q = std::min(t.mr, t.mc);
+        const Off& o = off[i];
         jac[i] = JacItem{};
-        jac[i].x = t.x;
-        jac[i].v = reinterpret_cast<cplx*>(ws + off_v[i]);
-        jac[i].p = p;
-        jac[i].q = q;
+        jac[i].x = o.pre ? P(o.yj) : t.x;
+        jac[i].v = P(o.v);
+        jac[i].p = o.pre ? o.q_ : o.p_;
+        jac[i].q = o.q_;
         SelectItem& s = sel[i];
         s = SelectItem{};
-        s.x = t.x;
-        s.p = p;
-        s.q = q;
-        s.sigma_sorted = reinterpret_cast<double*>(ws + off_s[i]);
-        s.perm = reinterpret_cast<int*>(ws + off_p[i]);
-        s.lam_out = t.lam_dst ? t.lam_dst : reinterpret_cast<double*>(ws + off_l[i]);
+        s.x = jac[i].x;
+        s.p = jac[i].p;
+        s.q = o.q_;
+        s.sigma_sorted = reinterpret_cast<double*>(ws + o.s);
+        s.perm = reinterpret_cast<int*>(ws + o.p);
+        s.lam_out = t.lam_dst ? t.lam_dst : reinterpret_cast<double*>(ws + o.l);
         s.kept = d_kept + i;
         s.discarded = d_disc + i;
         s.threshold = t.threshold;
@@ -391,38 +487,71 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     }
     run_jacobi(c, jac);
     run_select(c, sel);
+
+    // ---- preconditioned tasks: U_X = Q1 [W[:, perm] / σ ; 0],  V_X = Q2 V_J[:, perm]
+    std::vector<GatherItem> pre_g;
+    std::vector<QrApplyJob> app;
+    for (size_t i = 0; i < nt; ++i) {
+        const Off& o = off[i];
+        if (!o.pre) continue;
+        launch_fill(reinterpret_cast<double*>(P(o.ux)), 2 * (size_t)o.p_ * std::max(o.k_, 1), 0.0, c->stream);
+        GatherItem g{};
+        g.src = jac[i].x; g.ld_src = o.q_; g.rows = o.q_; g.perm = sel[i].perm; g.sigma_sorted = sel[i].sigma_sorted;
+        g.divide_sigma = 1; g.kept = sel[i].kept; g.rowscale_mod = 1; g.rowscale_div = 1;
+        g.out = P(o.ux); g.conj_transpose = 0; g.max_cols = o.k_; g.ld_out = o.p_;
+        pre_g.push_back(g);
+        GatherItem h{};
+        h.src = jac[i].v; h.ld_src = o.q_; h.rows = o.q_; h.perm = sel[i].perm; h.sigma_sorted = sel[i].sigma_sorted;
+        h.divide_sigma = 0; h.kept = sel[i].kept; h.rowscale_mod = 1; h.rowscale_div = 1;
+        h.out = P(o.vx); h.conj_transpose = 0; h.max_cols = o.k_;
+        pre_g.push_back(h);
+    }
+    run_gather(c, pre_g);
+    for (size_t i = 0; i < nt; ++i) {
+        if (!off[i].pre) continue;
+        app.push_back(QrApplyJob{&qr1[i], P(off[i].ux), off[i].k_});
+        app.push_back(QrApplyJob{&qr2[i], P(off[i].vx), off[i].k_});
+    }
+    run_qr_apply_q(c, app);
+
+    // ---- outputs
     std::vector<GatherItem> gat;
-    for (size_t i = 0; i < tasks.size(); ++i) {
+    for (size_t i = 0; i < nt; ++i) {
         const SvdTask& t = tasks[i];
+        const Off& o = off[i];
         const bool xh = t.mr < t.mc;  // X holds M^H
-        const int p = std::max(t.mr, t.mc), q = std::min(t.mr, t.mc);
-        for (const SvdOut& o : t.outs) {
+        for (const SvdOut& so : t.outs) {
             GatherItem g{};
-            // factor rows: U_M has mr rows, V_M has mc rows
-            // source: columns of X (normalised by σ) or of the Jacobi V
-            const bool from_x = (o.factor == 0) != xh;
-            g.src = from_x ? t.x : jac[i].v;
-            g.ld_src = from_x ? p : q;
-            g.rows = (o.factor == 0) ? t.mr : t.mc;
-            g.perm = sel[i].perm;
-            g.sigma_sorted = sel[i].sigma_sorted;
-            g.divide_sigma = from_x ? 1 : 0;
+            const bool from_x = (so.factor == 0) != xh;  // the factor is X's left factor
+            g.rows = (so.factor == 0) ? t.mr : t.mc;
             g.kept = sel[i].kept;
-            g.rowscale = o.rowscale;
-            g.rowscale_mod = o.rowscale_mod > 0 ? o.rowscale_mod : 1;
-            g.rowscale_div = o.rowscale_div > 0 ? o.rowscale_div : 1;
-            g.out = o.dst;
-            g.conj_transpose = o.conj_transpose;
-            g.max_cols = t.kmax();
+            g.sigma_sorted = sel[i].sigma_sorted;
+            g.rowscale = so.rowscale;
+            g.rowscale_mod = so.rowscale_mod > 0 ? so.rowscale_mod : 1;
+            g.rowscale_div = so.rowscale_div > 0 ? so.rowscale_div : 1;
+            g.out = so.dst;
+            g.conj_transpose = so.conj_transpose;
+            g.max_cols = o.k_;
+            if (o.pre) {
+                g.src = from_x ? P(o.ux) : P(o.vx);
+                g.ld_src = from_x ? o.p_ : o.q_;
+                g.perm = nullptr;
+                g.divide_sigma = 0;
+            } else {
+                g.src = from_x ? t.x : jac[i].v;
+                g.ld_src = from_x ? o.p_ : o.q_;
+                g.perm = sel[i].perm;
+                g.divide_sigma = from_x ? 1 : 0;
+            }
             gat.push_back(g);
         }
     }
     run_gather(c, gat);
-    std::vector<int> kept(tasks.size());
-    std::vector<double> disc(tasks.size());
+    std::vector<int> kept(nt);
+    std::vector<double> disc(nt);
     readback(c, kept.data(), d_kept, kept.size() * sizeof(int));
     readback(c, disc.data(), d_disc, disc.size() * sizeof(double));
-    for (size_t i = 0; i < tasks.size(); ++i) {
+    for (size_t i = 0; i < nt; ++i) {
         tasks[i].kept = kept[i];
         tasks[i].discarded = disc[i];
     }

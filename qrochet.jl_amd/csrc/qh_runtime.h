@@ -55,6 +55,8 @@ struct qh_context {
     int jac_block = 16;
     int jac_max_sweeps = 40;
     int jac_sort_sweeps = 2;  // sweeps that order eigenpairs by size (then plain cyclic)
+    int jac_precondition = 1; // QR-precondition SVDs with more than one block pair
+    int jac_max_inner = 2;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
 };
 
 struct qh_mps {
@@ -135,5 +137,22 @@ struct SvdTask {
 };
 // Runs the batch of SVD tasks (Jacobi + select + gathers) and synchronises.
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks);
+
+// Householder QR of every job in place (R in the upper triangle); reflectors into v (m × n),
+// compact-WY triangles into t (ceil(n / QR_NB) · QR_NB² entries).
+struct QrJob {
+    cplx* a;
+    int m, n, lda;
+    cplx* v;
+    cplx* t;
+};
+void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs);
+// y (m × ncols, ld m) <- Q y for every job (Q = the factorisation's orthogonal factor)
+struct QrApplyJob {
+    const QrJob* qr;
+    cplx* y;
+    int ncols;
+};
+void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs);
 
 }  // namespace qh
