@@ -134,10 +134,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         sQ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-#pragma unroll 1
     // one block pair per matrix: solve the block exactly; otherwise an approximate
     // diagonaliser suffices (the outer sweeps re-check every pair on fresh Grams)
     const int inner_cap = (it.nb == 2) ? 12 : max_inner;
+#pragma unroll 1
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         if (tid == 0) sFlag = 0;
         __syncthreads();
