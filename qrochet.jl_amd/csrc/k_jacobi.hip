@@ -66,13 +66,31 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
 #pragma unroll
         for (int b = 0; b < TS; ++b) acc[a][b] = cz();
 
-    for (int row0 = 0; row0 < p; row0 += CH) {
-        for (int e = tid; e < BC * CH; e += 256) {
+    // chunk loads are software-pipelined: chunk k+1 is fetched into registers
+    // while chunk k is reduced from LDS
+    constexpr int LPT = BC * CH / 256;  // elements per thread per chunk
+    double2 pre[LPT];
+    auto fetch = [&](const double2* base, int rows, int row0) {
+#pragma unroll
+        for (int s2 = 0; s2 < LPT; ++s2) {
+            const int e = tid + 256 * s2;
             const int c = e / CH, rr = e % CH;
             const int gc = sCol[c], grow = row0 + rr;
-            sX[c][rr] = (gc >= 0 && grow < p) ? it.x[grow + (size_t)p * gc] : cz();
+            pre[s2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
         }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int s2 = 0; s2 < LPT; ++s2) {
+            const int e = tid + 256 * s2;
+            sX[e / CH][e % CH] = pre[s2];
+        }
+    };
+    fetch(it.x, p, 0);
+    for (int row0 = 0; row0 < p; row0 += CH) {
+        stash();
         __syncthreads();
+        if (row0 + CH < p) fetch(it.x, p, row0 + CH);
 #pragma unroll 2
         for (int rr = wv; rr < CH; rr += 4) {
             double2 xi[TS], xj[TS];
@@ -256,13 +274,11 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     for (int mat = 0; mat < 2; ++mat) {
         double2* base = mat ? it.v : it.x;
         const int rows = mat ? it.q : p;
+        fetch(base, rows, 0);
         for (int row0 = 0; row0 < rows; row0 += CH) {
-            for (int e = tid; e < BC * CH; e += 256) {
-                const int c = e / CH, r2 = e % CH;
-                const int gc = sCol[c], grow = row0 + r2;
-                sX[c][r2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
-            }
+            stash();
             __syncthreads();
+            if (row0 + CH < rows) fetch(base, rows, row0 + CH);
             double2 o[JG];
 #pragma unroll
             for (int a = 0; a < JG; ++a) o[a] = cz();

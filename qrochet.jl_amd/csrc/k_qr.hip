@@ -39,15 +39,23 @@ __device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ items, int j0) {
-    __shared__ double2 sRed[4 * QR_NB];
-    __shared__ double2 sDot[QR_NB];
-    __shared__ double2 sT[QR_NB][QR_NB + 1];
-    __shared__ double2 sScale;
-    __shared__ double sBeta;
-    __shared__ double2 sTau[QR_NB];
-    const QrItem it = items[blockIdx.x];
-    if (j0 >= it.n) return;
+struct PanelShared {
+    double2 sRed[4 * QR_NB];
+    double2 sDot[QR_NB];
+    double2 sT[QR_NB][QR_NB + 1];
+    double2 sScale;
+    double sBeta;
+    double2 sTau[QR_NB];
+};
+
+// panel factorisation streaming from global memory (any number of rows)
+__device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
+    double2* sRed = sh.sRed;
+    double2* sDot = sh.sDot;
+    auto& sT = sh.sT;
+    double2& sScale = sh.sScale;
+    double& sBeta = sh.sBeta;
+    double2* sTau = sh.sTau;
     const int tid = threadIdx.x;
     const int m = it.m;
     const int nb = min(QR_NB, it.n - j0);
@@ -136,6 +144,111 @@ __global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ ite
         const int i = e % QR_NB, j = e / QR_NB;
         t[i + QR_NB * j] = (i < nb && j < nb) ? sT[i][j] : cz();
     }
+}
+
+
+constexpr int QR_LDS_ROWS = 512;  // panels of up to this many rows are factored in LDS
+
+// panel factorisation with the whole panel resident in LDS (rows j0..m-1 <= QR_LDS_ROWS)
+__device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 (*sP)[QR_LDS_ROWS]) {
+    const int tid = threadIdx.x;
+    const int m = it.m;
+    const int rows = m - j0;
+    const int nb = min(QR_NB, it.n - j0);
+    for (int e = tid; e < QR_NB * rows; e += 256) {
+        const int c = e / rows, r = e % rows;
+        sP[c][r] = (c < nb) ? it.a[(j0 + r) + (size_t)it.lda * (j0 + c)] : cz();
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        double2 part[1] = {cz()};
+        for (int r = c + 1 + tid; r < rows; r += 256) part[0].x += cabs2(sP[c][r]);
+        block_sum<1>(part, sh.sRed, sh.sDot);
+        if (tid == 0) {
+            const double xnorm2 = sh.sDot[0].x;
+            const double2 alpha = sP[c][c];
+            double2 tau = cz(), scale = cz();
+            double beta = alpha.x;
+            if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+                const double nrm = sqrt(cabs2(alpha) + xnorm2);
+                beta = alpha.x >= 0.0 ? -nrm : nrm;
+                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+                const double2 den = make_double2(alpha.x - beta, alpha.y);
+                const double dd = cabs2(den);
+                scale = make_double2(den.x / dd, -den.y / dd);
+            }
+            sh.sTau[c] = tau;
+            sh.sScale = scale;
+            sP[c][c] = make_double2(beta, 0.0);
+        }
+        __syncthreads();
+        const double2 scale = sh.sScale;
+        for (int r = c + 1 + tid; r < rows; r += 256) sP[c][r] = cmul(sP[c][r], scale);  // v below the diagonal
+        __syncthreads();
+        // dots with v = (1 at r = c, sP[c][r] below)
+        double2 dots[QR_NB];
+#pragma unroll
+        for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
+        for (int r = c + tid; r < rows; r += 256) {
+            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : sP[c][r];
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) {
+                if (k > c && k < nb) dots[k] = cfmac(vr, sP[k][r], dots[k]);
+                else if (k < c) {
+                    const double2 vk = sP[k][r];  // r >= c > k: below column k's diagonal
+                    dots[k] = cfmac(vk, vr, dots[k]);
+                }
+            }
+        }
+        block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
+        if (tid < QR_NB) {
+            const int i = tid;
+            double2 val = cz();
+            if (i < c) {
+                double2 acc = cz();
+                for (int l = i; l < c; ++l) acc = cfma(sh.sT[i][l], sh.sDot[l], acc);
+                const double2 tc = sh.sTau[c];
+                val = make_double2(-(tc.x * acc.x - tc.y * acc.y), -(tc.x * acc.y + tc.y * acc.x));
+            } else if (i == c) {
+                val = sh.sTau[c];
+            }
+            sh.sT[i][c] = val;
+        }
+        const double2 ctau = cconj(sh.sTau[c]);
+        for (int e = tid; e < (nb - c - 1) * (rows - c); e += 256) {
+            const int k = c + 1 + e / (rows - c), r = c + e % (rows - c);
+            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : sP[c][r];
+            sP[k][r] = csub(sP[k][r], cmul(cmul(ctau, sh.sDot[k]), vr));
+        }
+        __syncthreads();
+    }
+    // write R (upper part) back to a, the explicit reflectors to v
+    for (int e = tid; e < nb * rows; e += 256) {
+        const int c = e / rows, r = e % rows;
+        const size_t ai = (j0 + r) + (size_t)it.lda * (j0 + c);
+        const size_t vi = (j0 + r) + (size_t)it.m * (j0 + c);
+        if (r <= c) {
+            it.a[ai] = sP[c][r];
+            it.v[vi] = (r == c) ? make_double2(1.0, 0.0) : cz();
+        } else {
+            it.a[ai] = cz();
+            it.v[vi] = sP[c][r];
+        }
+    }
+    double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
+    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+        const int i = e % QR_NB, j = e / QR_NB;
+        t[i + QR_NB * j] = (i < nb && j < nb) ? sh.sT[i][j] : cz();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ items, int j0) {
+    __shared__ PanelShared sh;
+    __shared__ double2 sP[QR_NB][QR_LDS_ROWS];
+    const QrItem it = items[blockIdx.x];
+    if (j0 >= it.n) return;
+    if (it.m - j0 <= QR_LDS_ROWS) qr_panel_lds(it, j0, sh, sP);
+    else qr_panel_global(it, j0, sh);
 }
 
 // Y[j0:, cols] <- (I - V T' V^H) Y[j0:, cols],  T' = T^H (adjoint) or T

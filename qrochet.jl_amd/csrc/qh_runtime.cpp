@@ -235,12 +235,17 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     bool done = false;
     for (int sweep = 0; sweep < c->jac_max_sweeps; ++sweep) {
         // read the per-item rotation counters after the window to account work
-        for (int r = 0; r < rmax; ++r) {
-            const int g = sweep * rmax + r;
+        {
+            // one event pair per window of back-to-back rounds (per-launch events would
+            // cost more host time than the short rounds themselves)
             Timed tm(c, "jacobi_round");
+            if (c->timing) c->timers["jacobi_round"].launches += rmax - 1;
             static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
-            launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, floor_rel2, sweep < c->jac_sort_sweeps ? 1 : 0,
-                                kdbg, c->jac_max_inner, c->stream);
+            for (int r = 0; r < rmax; ++r) {
+                const int g = sweep * rmax + r;
+                launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, floor_rel2,
+                                    sweep < c->jac_sort_sweeps ? 1 : 0, kdbg, c->jac_max_inner, c->stream);
+            }
         }
         static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
         if (dbg) {

@@ -56,7 +56,7 @@ struct qh_context {
     int jac_max_sweeps = 40;
     int jac_sort_sweeps = 2;  // sweeps that order eigenpairs by size (then plain cyclic)
     int jac_precondition = 1; // QR-precondition SVDs with more than one block pair
-    int jac_max_inner = 2;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
+    int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
 };
 
 struct qh_mps {

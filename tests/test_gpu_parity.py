@@ -198,3 +198,153 @@ def test_ising_trotter_prefix_parity(qh, batched):
     z_gpu = qh.expect_many(psi, zs)
     z_ref = np.array([oc.expect(ref, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)])
     assert np.max(np.abs(z_gpu - z_ref)) < TOL
+
+
+def test_config1_matches_golden_fixture(qh):
+    from tests.golden import make_golden
+
+    fx = np.load(make_golden.path("config1"), allow_pickle=False)
+    arrays, schedule, tr = wl.config1()
+    psi = qh.from_internal(arrays)
+    psi0 = psi.copy()
+    kept = []
+    for b, g in schedule:
+        info = {}
+        qh.evolve_(psi, qh.gate(g, b, b + 1), maxdim=tr["maxdim"], info=info)
+        kept.append(info["kept"])
+    assert np.array_equal(np.array(kept), fx["kept"])
+    z = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in range(1, 17)])
+    assert np.max(np.abs(z - fx["z"])) < TOL
+    assert abs(qh.overlap(psi, psi0) - fx["overlap"]) < TOL
+
+
+def test_canonize_site_reference(qh):
+    """`Chain_test.jl:268-306` through the C ABI."""
+    rng = np.random.default_rng(21)
+    arrays = [rng.random((4, 4)), rng.random((4, 4, 4)), rng.random((4, 4))]
+    qtn = qh.MPS(arrays)
+    v0 = oc.statevector(oc.MPS(arrays))
+    for method in ("qr", "svd"):
+        for site, direction, pred in ((1, "right", qh.isleftcanonical), (2, "right", qh.isleftcanonical),
+                                      (2, "left", qh.isrightcanonical), (3, "left", qh.isrightcanonical)):
+            c = qh.canonize_site(qtn, site, direction=direction, method=method)
+            assert pred(c, site), (method, site, direction)
+            assert np.max(np.abs(_state(c) - v0)) < 1e-12 * np.max(np.abs(v0))
+    assert qh.canonize_site(qtn, 2, direction="left", method="svd").ntensors() == 4
+    with pytest.raises(qh.ArgumentError):
+        qh.canonize_site_(qtn, 2, direction="left", method="lu")
+    with pytest.raises(qh.ArgumentError):
+        qh.canonize_site_(qtn, 2, direction="up")
+
+
+def test_mixed_canonize_normalize_reference(qh):
+    """`Chain_test.jl:358-381` through the C ABI."""
+    rng = np.random.default_rng(22)
+    arrays = [rng.random((4, 4))] + [rng.random((4, 4, 4)) for _ in range(3)] + [rng.random((4, 4))]
+    qtn = qh.MPS(arrays)
+    c = qh.mixed_canonize(qtn, 3)
+    assert c.ntensors() == qtn.ntensors() + 1
+    assert qh.isleftcanonical(c, 1) and qh.isleftcanonical(c, 2)
+    assert all(qh.isrightcanonical(c, i) for i in (3, 4, 5))
+    v0 = oc.statevector(oc.MPS(arrays))
+    assert np.max(np.abs(_state(c) - v0)) < 1e-12 * np.max(np.abs(v0))
+    m = qh.MPS(arrays)
+    qh.normalize_(m, 3)
+    assert abs(qh.norm(m) - 1.0) < 1e-12
+    ref = oc.normalize(oc.MPS(arrays), 3)
+    assert np.max(np.abs(m.lambda_((2, 3)) - ref.lam[1])) < 1e-12
+
+
+def test_contract_between_reference(qh):
+    """`Chain_test.jl:241-266` through the C ABI."""
+    rng = np.random.default_rng(23)
+    ref = oc.rand_chain(5, 20, rng=rng)
+    qtn = qh.from_internal(ref.arrays)
+    can = qh.canonize(qtn)
+    with pytest.raises(qh.ArgumentError):
+        qh.contract_between_(can.copy(), 1, 2, direction="dummy")
+    v0 = oc.statevector(ref)
+    for i in range(1, 5):
+        some = qh.contract_between(can, i, i + 1)
+        assert np.max(np.abs(_state(some) - v0)) < 1e-12
+        assert qh.isrightcanonical(some, i)
+        assert qh.isleftcanonical(qh.contract_between(can, i, i + 1, direction="right"), i + 1)
+        g = can.internal_arrays()[i - 1] * can.lambda_(i)[None, None, :]
+        assert np.max(np.abs(some.internal_arrays()[i - 1] - g)) < 1e-13
+
+
+def test_expect_overlap_norm(qh):
+    rng = np.random.default_rng(24)
+    n = 7
+    a_arr = _random_chain(rng, n, 2, 8, canon=False)
+    b_arr = _random_chain(rng, n, 2, 4, canon=False)
+    lams = [rng.random(d) + 0.1 for d in [a.shape[2] for a in a_arr[:-1]]]
+    a = qh.from_internal(a_arr, lambdas=lams)
+    b = qh.from_internal(b_arr)
+    ra, rb = oc.Chain([x.copy() for x in a_arr], [l.copy() for l in lams]), oc.Chain([x.copy() for x in b_arr])
+    va, vb = oc.statevector(ra), oc.statevector(rb)
+    scale = np.max(np.abs(va)) * np.max(np.abs(vb)) * va.size
+    assert abs(qh.overlap(a, b) - np.vdot(vb, va)) < 1e-12 * scale
+    assert abs(qh.norm(a) - np.linalg.norm(va)) < 1e-12 * np.linalg.norm(va)
+    g = wl.haar_gate4(rng)
+    nv = np.vdot(va, va).real
+    assert abs(qh.expect(a, [qh.gate(g, 3, 4)]) - sv.expect2(va, g, 3)) < 1e-12 * nv
+    # products in list order, overlapping supports, both on one cluster and apart
+    want = np.vdot(va, sv.apply1(sv.apply1(va, wl.X, 2), wl.Z, 5))
+    assert abs(qh.expect(a, [qh.gate(wl.X, 2), qh.gate(wl.Z, 5)]) - want) < 1e-12 * nv
+    want = np.vdot(va, sv.apply1(sv.apply2(va, g, 3), wl.Y, 4))
+    assert abs(qh.expect(a, [qh.gate(g, 3, 4), qh.gate(wl.Y, 4)]) - want) < 1e-12 * nv
+    want = np.vdot(va, sv.apply2(sv.apply2(va, g, 2), g, 3))
+    assert abs(qh.expect(a, [qh.gate(g, 2, 3), qh.gate(g, 3, 4)]) - want) < 1e-12 * nv
+    terms = [[qh.gate(wl.Z, i)] for i in range(1, n + 1)] + [[qh.gate(wl.Z, i), qh.gate(wl.Z, j)]
+                                                              for i in range(1, n) for j in range(i + 1, min(n, i + 4) + 1)]
+    got = qh.expect_many(a, terms)
+    wants = [sv.expect1(va, wl.Z, i) for i in range(1, n + 1)]
+    wants += [np.vdot(va, sv.apply1(sv.apply1(va, wl.Z, i), wl.Z, j)) for i in range(1, n)
+              for j in range(i + 1, min(n, i + 4) + 1)]
+    assert np.max(np.abs(got - np.array(wants))) < 1e-12 * nv
+
+
+def test_evolve1_and_renormalize(qh):
+    rng = np.random.default_rng(25)
+    n = 6
+    arrays = _random_chain(rng, n, 2, 4)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    u = wl.haar_unitary(rng, 2)
+    qh.evolve_(psi, qh.gate(u, 3))
+    oc.evolve(ref, oc.gate1(u, 3))
+    assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-13
+    # Vidal mode with renormalisation: Λ normalised on the updated bond
+    qh.canonize_(psi)
+    oc.canonize(ref)
+    g = wl.haar_gate4(rng)
+    qh.simple_update_(psi, qh.gate(g, 3, 4), maxdim=2, renormalize=True)
+    oc.simple_update(ref, oc.gate2(g, 3), maxdim=2, renormalize=True)
+    assert abs(np.linalg.norm(psi.lambda_(3)) - 1) < 1e-13
+    assert np.max(np.abs(psi.lambda_(3) - ref.lam[2])) < 1e-12
+    # mode A renormalisation re-canonises around the left site of the bond
+    a = qh.from_internal(arrays)
+    r = oc.Chain([x.copy() for x in arrays])
+    qh.evolve_(a, qh.gate(g, 3, 4), maxdim=2, renormalize=True)
+    oc.evolve(r, oc.gate2(g, 3), maxdim=2, renormalize=True)
+    assert a.has_lambda() == [lam is not None for lam in r.lam]
+    assert np.max(np.abs(_state(a) - oc.statevector(r))) < 1e-12
+    with pytest.raises(qh.ArgumentError):
+        qh.evolve_(qh.from_internal(arrays), qh.gate(g, 1, 2), maxdim=2, renormalize=True)
+
+
+def test_rank_deficient_theta_without_truncation(qh):
+    """θ with exactly dependent columns (middle bond 1, outer bonds 4): the
+    untruncated update keeps all min(m, n) singular values like zgesdd and the
+    state stays exact."""
+    rng = np.random.default_rng(26)
+    arrays = [rng.standard_normal((1, 2, 2)) + 0j, rng.standard_normal((2, 2, 1)) + 0j,
+              rng.standard_normal((1, 2, 2)) + 0j, rng.standard_normal((2, 2, 1)) + 0j]
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    g = wl.haar_gate4(rng)
+    qh.evolve_(psi, qh.gate(g, 2, 3))
+    oc.evolve(ref, oc.gate2(g, 2))
+    assert psi.bond_dims() == ref.bond_dims()
+    assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-12

@@ -1,0 +1,397 @@
+"""CPU checks that pin the oracle (oracle/chain.py) before it is trusted.
+
+Each oracle function is checked against something that fixes it independently:
+brute-force statevectors, closed forms, the invariants of the reference's own
+tests (test/Ansatz/Chain_test.jl, cited per test), and gate-convention checks
+chosen so that a transposed gate, swapped sites or a dropped Λ would fail.
+"""
+import numpy as np
+import pytest
+
+from oracle import chain as oc
+from oracle import statevector as sv
+from oracle import workloads as wl
+
+X, Y, Z, I2 = wl.X, wl.Y, wl.Z, wl.I2
+
+
+def rand_chain(rng, n, d=2, chi=4, canon=False):
+    arrays = wl.random_mps_arrays(rng, n, d, wl.max_profile(n, d, chi))
+    if canon:
+        arrays = wl.left_canonize_normalize(arrays)
+    return oc.Chain(arrays)
+
+
+# ---------------------------------------------------------------------------
+# statevector module: closed forms
+# ---------------------------------------------------------------------------
+def test_statevector_closed_forms():
+    zero, one = np.array([1, 0]), np.array([0, 1])
+    psi = sv.product_state([zero, one, zero])
+    assert sv.expect1(psi, Z, 1) == pytest.approx(1)
+    assert sv.expect1(psi, Z, 2) == pytest.approx(-1)
+    flipped = sv.apply1(psi, X, 1)
+    assert sv.expect1(flipped, Z, 1) == pytest.approx(-1)
+    # CNOT (control = left site) on |+>|0> gives a Bell state
+    cnot = np.zeros((4, 4))
+    for a in range(2):
+        for b in range(2):
+            cnot[a + 2 * (b ^ a), a + 2 * b] = 1  # row/col index: left site fastest
+    plus = np.array([1, 1]) / np.sqrt(2)
+    bell = sv.apply2(sv.product_state([plus, zero]), wl.as_gate4(cnot), 1)
+    assert np.allclose(bell, np.array([[1, 0], [0, 1]]) / np.sqrt(2))
+
+
+def test_gate4_site_convention():
+    """as_gate4(kron(B, A)) acts as A on the left site and B on the right site:
+    the left site is the fastest index of the 4×4 matrix (Julia reshape)."""
+    rng = np.random.default_rng(0)
+    A, B = wl.haar_unitary(rng, 2), wl.haar_unitary(rng, 2)
+    psi = rng.standard_normal((2, 2, 2)) + 1j * rng.standard_normal((2, 2, 2))
+    got = sv.apply2(psi, wl.as_gate4(np.kron(B, A)), 2)
+    want = sv.apply1(sv.apply1(psi, A, 2), B, 3)
+    assert np.allclose(got, want)
+    # a swapped-site gate must not pass
+    wrong = sv.apply2(psi, wl.as_gate4(np.kron(A, B)), 2)
+    assert not np.allclose(wrong, want)
+
+
+# ---------------------------------------------------------------------------
+# oracle vs statevector (no truncation -> exact)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,chi,seed", [(4, 2, 0), (6, 4, 1), (8, 8, 2)])
+def test_evolve_matches_statevector(n, chi, seed):
+    rng = np.random.default_rng(seed)
+    psi = rand_chain(rng, n, chi=chi)
+    v = oc.statevector(psi)
+    for layer in range(4):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            oc.evolve(psi, oc.gate2(g, b))
+            v = sv.apply2(v, g, b)
+        s = rng.integers(1, n + 1)
+        u = wl.haar_unitary(rng, 2)
+        oc.evolve(psi, oc.gate1(u, s))
+        v = sv.apply1(v, u, s)
+    assert np.max(np.abs(oc.statevector(psi) - v)) < 1e-11 * np.max(np.abs(v))
+
+
+def test_gate_given_in_other_site_order():
+    """A gate whose `sites` list is [i+1, i, (i+1)', i'] is the same operator."""
+    rng = np.random.default_rng(3)
+    psi = rand_chain(rng, 5)
+    g = wl.haar_gate4(rng)
+    a = oc.evolve(psi.copy(), oc.gate2(g, 2))
+    swapped = oc.Dense(np.transpose(g, (1, 0, 3, 2)), [oc.Site(3), oc.Site(2), oc.Site(3, True), oc.Site(2, True)])
+    b = oc.evolve(psi.copy(), swapped)
+    assert np.allclose(oc.statevector(a), oc.statevector(b))
+    # out/in transposed (G^T instead of G) is a different operator
+    bad = oc.evolve(psi.copy(), oc.gate2(np.transpose(g, (2, 3, 0, 1)), 2))
+    assert not np.allclose(oc.statevector(a), oc.statevector(bad))
+
+
+def test_mode_a_theta_includes_own_lambda_only():
+    """Mode A contracts the bond's own Λ into θ (`contract!(tn, bond)`), not the
+    neighbouring ones; the state (all Λ's included) is still exact."""
+    rng = np.random.default_rng(4)
+    psi = rand_chain(rng, 6)
+    psi.lam = [rng.random(d) + 0.5 for d in psi.bond_dims()]
+    v = oc.statevector(psi)
+    th = oc.theta(psi, 3)
+    want = np.einsum("lam,m,mbr->labr", psi.arrays[2], psi.lam[2], psi.arrays[3])
+    assert np.allclose(th, want)
+    g = wl.haar_gate4(rng)
+    oc.evolve(psi, oc.gate2(g, 3))
+    assert np.allclose(oc.statevector(psi), sv.apply2(v, g, 3))
+
+
+def test_vidal_mode_exact_without_truncation():
+    rng = np.random.default_rng(5)
+    psi = rand_chain(rng, 7, chi=8)
+    oc.canonize(psi)
+    v = oc.statevector(psi)
+    for layer in range(4):
+        for b in wl.brickwork_bonds(7, layer):
+            g = wl.haar_gate4(rng)
+            oc.simple_update(psi, oc.gate2(g, b))
+            v = sv.apply2(v, g, b)
+    assert np.max(np.abs(oc.statevector(psi) - v)) < 1e-10 * np.max(np.abs(v))
+
+
+# ---------------------------------------------------------------------------
+# closed forms of the Ising Trotter workload
+# ---------------------------------------------------------------------------
+def test_ising_h_only_closed_form():
+    """J = 0: commuting single-site X rotations; from |0…0⟩, ⟨Z_i⟩ = cos(2θ_i) with
+    θ_i = Σ_layers Σ_{bonds b ∋ i} (h/2)·τ."""
+    n, dt, steps, h = 8, 0.05, 4, 0.9
+    arrays = [np.array([1.0, 0.0], dtype=complex).reshape(1, 2, 1) for _ in range(n)]
+    psi = oc.Chain(arrays)
+    theta = np.zeros(n)
+    for _ in range(steps):
+        for parity, tau in ((0, dt / 2), (1, dt), (0, dt / 2)):
+            for b in wl.brickwork_bonds(n, parity):
+                oc.evolve(psi, oc.gate2(wl.ising_gate(0.0, h, tau), b), threshold=1e-12, maxdim=64)
+                theta[b - 1] += 0.5 * h * tau
+                theta[b] += 0.5 * h * tau
+    z = np.array([oc.expect(psi, [oc.gate1(Z, i)]).real for i in range(1, n + 1)])
+    assert np.allclose(z, np.cos(2 * theta), atol=1e-12)
+    assert max(psi.bond_dims()) == 1  # a product state stays a product state
+
+
+def test_ising_j_only_keeps_z():
+    """h = 0: diagonal gates, |amplitudes| and ⟨Z_i⟩ constant."""
+    rng = np.random.default_rng(6)
+    n = 6
+    arrays = []
+    for _ in range(n):
+        v = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+        arrays.append((v / np.linalg.norm(v)).reshape(1, 2, 1))
+    psi = oc.Chain(arrays)
+    z0 = [oc.expect(psi, [oc.gate1(Z, i)]).real for i in range(1, n + 1)]
+    for layer in range(6):
+        for b in wl.brickwork_bonds(n, layer):
+            oc.evolve(psi, oc.gate2(wl.ising_gate(1.1, 0.0, 0.05), b), threshold=1e-12, maxdim=16)
+    z1 = [oc.expect(psi, [oc.gate1(Z, i)]).real for i in range(1, n + 1)]
+    assert np.allclose(z0, z1, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# truncation rule  (Chain.jl:390-422, Chain_test.jl:189-204)
+# ---------------------------------------------------------------------------
+def _chain_with_lambda(lam):
+    rng = np.random.default_rng(7)
+    k = len(lam)
+    a = np.linalg.qr(rng.standard_normal((2, 2)))[0].reshape(1, 2, 2)
+    b = rng.standard_normal((2, 2, k)) + 0j
+    c = rng.standard_normal((k, 2, 1)) + 0j
+    psi = oc.Chain([a.astype(complex), b, c])
+    psi.lam[1] = np.array(lam, dtype=float)
+    return psi
+
+
+def test_truncate_rule():
+    lam = [0.9, 0.4, 0.1, 1e-17]
+    assert oc.truncate(_chain_with_lambda(lam), (2, 3), maxdim=2).lam[1].tolist() == [0.9, 0.4]
+    assert oc.truncate(_chain_with_lambda(lam), (2, 3), threshold=0.2).lam[1].tolist() == [0.9, 0.4]
+    # default threshold 1e-16 drops the last value when only maxdim is given
+    assert oc.truncate(_chain_with_lambda(lam), (2, 3), maxdim=10).lam[1].tolist() == [0.9, 0.4, 0.1]
+    # absolute, not squared and not relative
+    assert len(oc.truncate(_chain_with_lambda(lam), (2, 3), threshold=0.1).lam[1]) == 2
+    # filter applies after the maxdim cut, and keeps non-prefix survivors
+    p = _chain_with_lambda([0.9, 1e-20, 0.5, 0.4])
+    assert oc.truncate(p, (2, 3), maxdim=3).lam[1].tolist() == [0.9, 0.5]
+    assert oc.truncated_weight(np.array(lam), maxdim=2) == pytest.approx(np.sqrt(0.1**2 + 1e-34))
+
+
+def test_reference_truncate_test():
+    """Chain_test.jl:189-204."""
+    rng = np.random.default_rng(8)
+    qtn = oc.MPS([rng.random((2, 2)), rng.random((2, 2, 2)), rng.random((2, 2))])
+    oc.canonize_site(qtn, 2, direction="right", method="svd")
+    with pytest.raises(oc.MissingSchmidtCoefficientsException):
+        oc.truncate(qtn.copy(), (1, 2), maxdim=1)
+    t = oc.truncate_copy(qtn, (2, 3), maxdim=1)
+    assert t.bond_dims()[1] == 1
+    s = qtn.lam[1]
+    t = oc.truncate_copy(qtn, (2, 3), threshold=s[1] + 0.1)
+    assert t.bond_dims()[1] == 1
+    with pytest.raises(oc.ArgumentError):
+        oc.truncate(qtn.copy(), (2, 1), maxdim=1)
+
+
+def test_svd_truncation_is_eckart_young():
+    """After a truncated update, A_l Λ A_r is the best rank-k approximation of the
+    gated θ: ||θ' - θ_k||_F = sqrt(Σ dropped σ²) (Eckart–Young)."""
+    rng = np.random.default_rng(9)
+    psi = rand_chain(rng, 6, chi=8)
+    g = wl.haar_gate4(rng)
+    th = oc.theta(psi, 3, g)
+    s = np.linalg.svd(th.reshape(th.shape[0] * 2, -1), compute_uv=False)
+    cut = oc.evolve(psi.copy(), oc.gate2(g, 3), maxdim=3)
+    assert cut.bond_dims()[2] == 3
+    thk = oc.theta(cut, 3)
+    err = np.linalg.norm(th - thk)
+    assert err == pytest.approx(np.sqrt(np.sum(s[3:] ** 2)), rel=1e-10)
+    assert oc.truncated_weight(s, maxdim=3) == pytest.approx(err, rel=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# canonical forms  (Chain_test.jl:241-381)
+# ---------------------------------------------------------------------------
+def _ref_chain(rng, n=5, d=4):
+    return oc.MPS([rng.random((d, d))] + [rng.random((d, d, d)) for _ in range(n - 2)] + [rng.random((d, d))])
+
+
+def test_canonize_site_reference():
+    """Chain_test.jl:268-306."""
+    rng = np.random.default_rng(10)
+    qtn = oc.MPS([rng.random((4, 4)), rng.random((4, 4, 4)), rng.random((4, 4))])
+    with pytest.raises(oc.ArgumentError):
+        oc.canonize_site(qtn.copy(), 1, direction="left")
+    with pytest.raises(oc.ArgumentError):
+        oc.canonize_site(qtn.copy(), 3, direction="right")
+    v0 = oc.statevector(qtn)
+    for method in ("qr", "svd"):
+        for site, direction, pred in ((1, "right", oc.isleftcanonical), (2, "right", oc.isleftcanonical),
+                                      (2, "left", oc.isrightcanonical), (3, "left", oc.isrightcanonical)):
+            c = oc.canonize_site_copy(qtn, site, direction=direction, method=method)
+            assert pred(c, site)
+            assert np.allclose(oc.statevector(c), v0)
+    assert oc.canonize_site_copy(qtn, 2, direction="left", method="svd").ntensors() == 4
+    with pytest.raises(oc.ArgumentError):
+        oc.canonize_site(qtn.copy(), 2, direction="left", method="lu")
+
+
+def test_canonize_reference():
+    """Chain_test.jl:308-356."""
+    rng = np.random.default_rng(11)
+    qtn = _ref_chain(rng)
+    can = oc.canonize_copy(qtn)
+    assert can.ntensors() == 9
+    assert np.allclose(oc.statevector(can), oc.statevector(qtn))
+    nrm = oc.norm(can)
+    assert nrm == pytest.approx(oc.norm(qtn))
+    for lam in can.lam:
+        assert np.sum(lam**2) == pytest.approx(nrm**2)
+    for i in range(1, 6):
+        c = oc.canonize_copy(qtn)
+        if i == 1:
+            assert oc.isleftcanonical(c, i)
+        else:
+            oc.contract_between(c, i - 1, i, direction="right")
+            if i == 5:
+                c.arrays[i - 1] = c.arrays[i - 1] / nrm
+            assert oc.isleftcanonical(c, i)
+    for i in range(1, 6):
+        c = oc.canonize_copy(qtn)
+        if i == 5:
+            assert oc.isrightcanonical(c, i)
+        else:
+            oc.contract_between(c, i, i + 1, direction="left")
+            if i == 1:
+                c.arrays[0] = c.arrays[0] / nrm
+            assert oc.isrightcanonical(c, i)
+
+
+def test_contract_between_reference():
+    """Chain_test.jl:241-266 (on a complex random chain)."""
+    rng = np.random.default_rng(12)
+    qtn = oc.rand_chain(5, 20, rng=rng)
+    can = oc.canonize_copy(qtn)
+    with pytest.raises(oc.ArgumentError):
+        oc.contract_between(can.copy(), 1, 2, direction="dummy")
+    v0 = oc.statevector(qtn)
+    for i in range(1, 5):
+        some = oc.contract_between_copy(can, i, i + 1)
+        assert np.allclose(oc.statevector(some), v0)
+        assert oc.isrightcanonical(some, i)
+        assert oc.isleftcanonical(oc.contract_between_copy(can, i, i + 1, direction="right"), i + 1)
+        assert np.allclose(some.arrays[i - 1], can.arrays[i - 1] * can.lam[i - 1][None, None, :])
+
+
+def test_mixed_canonize_and_normalize_reference():
+    """Chain_test.jl:358-381."""
+    rng = np.random.default_rng(13)
+    qtn = _ref_chain(rng)
+    c = oc.mixed_canonize_copy(qtn, 3)
+    assert c.ntensors() == qtn.ntensors() + 1
+    assert oc.isleftcanonical(c, 1) and oc.isleftcanonical(c, 2)
+    assert all(oc.isrightcanonical(c, i) for i in (3, 4, 5))
+    assert np.allclose(oc.statevector(c), oc.statevector(qtn))
+    m = _ref_chain(rng)
+    oc.normalize(m, 3)
+    assert oc.norm(m) == pytest.approx(1.0)
+    with pytest.raises(oc.ArgumentError):
+        oc.normalize(_ref_chain(rng), 1)
+
+
+def test_rand_sampler_reference():
+    """Chain_test.jl:206-236 (State)."""
+    qtn = oc.rand_chain(8, 10, rng=np.random.default_rng(14))
+    assert oc.norm(qtn) == pytest.approx(1.0)
+    assert max(qtn.bond_dims()) <= 10
+    assert qtn.n == 8
+
+
+# ---------------------------------------------------------------------------
+# contractions  (Chain.jl:724-752, Ansatz.jl:101-109)
+# ---------------------------------------------------------------------------
+def test_expect_overlap_norm_vs_statevector():
+    rng = np.random.default_rng(15)
+    a = rand_chain(rng, 7, chi=8)
+    a.lam = [rng.random(d) + 0.1 for d in a.bond_dims()]
+    b = rand_chain(rng, 7, chi=4)
+    va, vb = oc.statevector(a), oc.statevector(b)
+    assert oc.overlap(a, b) == pytest.approx(np.vdot(vb, va))  # Σ a·conj(b) = ⟨b|a⟩
+    assert oc.overlap(b, a) == pytest.approx(np.conj(oc.overlap(a, b)))
+    assert oc.norm(a) == pytest.approx(np.linalg.norm(va))
+    for i in range(1, 8):
+        assert oc.expect(a, [oc.gate1(Z, i)]) == pytest.approx(sv.expect1(va, Z, i))
+    g = wl.haar_gate4(rng)
+    assert oc.expect(a, [oc.gate2(g, 3)]) == pytest.approx(sv.expect2(va, g, 3))
+    # a product of observables is applied in list order
+    want = np.vdot(va, sv.apply1(sv.apply1(va, X, 2), Z, 5))
+    assert oc.expect(a, [oc.gate1(X, 2), oc.gate1(Z, 5)]) == pytest.approx(want)
+
+
+def test_expect_after_truncated_evolution_is_consistent():
+    """⟨Z_i⟩ of the oracle equals the statevector of the oracle's own MPS."""
+    arrays, schedule, tr = wl.config1(n=8, chi=4, layers=4)
+    psi = oc.Chain([a.copy() for a in arrays])
+    for b, g in schedule:
+        oc.evolve(psi, oc.gate2(g, b), maxdim=4)
+    v = oc.statevector(psi)
+    for i in range(1, 9):
+        assert oc.expect(psi, [oc.gate1(Z, i)]) == pytest.approx(sv.expect1(v, Z, i))
+
+
+# ---------------------------------------------------------------------------
+# errors of evolve!  (Chain.jl:551-581)
+# ---------------------------------------------------------------------------
+def test_evolve_validation():
+    rng = np.random.default_rng(16)
+    psi = rand_chain(rng, 5)
+    g = wl.haar_gate4(rng)
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.Dense(g, [oc.Site(1), oc.Site(3), oc.Site(1, True), oc.Site(3, True)]))
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.Dense(g, [oc.Site(1), oc.Site(2), oc.Site(1, True), oc.Site(2, True)], socket="state"))
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.Dense(g, [oc.Site(1), oc.Site(2), oc.Site(1, True), oc.Site(3, True)]))
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.gate2(g, 5))
+    g3 = np.eye(8).reshape((2,) * 6)
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.Dense(g3, [oc.Site(1), oc.Site(2), oc.Site(3), oc.Site(1, True), oc.Site(2, True),
+                                     oc.Site(3, True)]))
+    # renormalize in mode A calls normalize!(ψ, bond[1]) which refuses Site(1)
+    with pytest.raises(oc.ArgumentError):
+        oc.evolve(psi, oc.gate2(g, 1), maxdim=4, renormalize=True)
+
+
+# ---------------------------------------------------------------------------
+# golden fixtures
+# ---------------------------------------------------------------------------
+def test_golden_config1_fixture():
+    from tests.golden import make_golden
+
+    fx = np.load(make_golden.path("config1"), allow_pickle=False)
+    got = make_golden.config1_outputs()
+    assert got["input_checksum"] == pytest.approx(fx["input_checksum"], rel=1e-14)
+    assert np.array_equal(got["kept"], fx["kept"])
+    assert np.max(np.abs(got["z"] - fx["z"])) < 1e-12
+    assert abs(got["overlap"] - fx["overlap"]) < 1e-12
+
+
+def test_golden_statevector_fixture():
+    from tests.golden import make_golden
+
+    fx = np.load(make_golden.path("small_exact"), allow_pickle=False)
+    psi = oc.Chain([fx[f"site{i}"] for i in range(int(fx["n"]))])
+    v = oc.statevector(psi)
+    for k in range(int(fx["ngates"])):
+        oc.evolve(psi, oc.gate2(fx[f"gate{k}"], int(fx["bonds"][k])))
+        v = sv.apply2(v, fx[f"gate{k}"], int(fx["bonds"][k]))
+    assert np.max(np.abs(oc.statevector(psi) - fx["final_state"])) < 1e-12
+    assert np.max(np.abs(v - fx["final_state"])) < 1e-12
