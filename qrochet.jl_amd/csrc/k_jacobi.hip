@@ -24,8 +24,8 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
 
 template <int BC>
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
-                                                      const int2* __restrict__ tiles, int round, double tol,
-                                                      double floor_rel2, int sort, int debug, int max_inner) {
+                                                      const int2* __restrict__ tiles, int round, int sort_sweeps,
+                                                      int debug, int max_inner) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
@@ -46,6 +46,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     if (*((volatile int*)it.conv)) return;
     const int R = it.nb - 1;
     const int r = round % R;
+    const double tol = it.tol;
+    const double floor_rel2 = it.floor_rel2;
+    // eigenpair sorting during this matrix's own first sweeps (independent of the batch)
+    const int sort = round < sort_sweeps * R;
     int bi, bj;
     rr_pair(r, tt.y, it.nb, bi, bj);
     if (tid < BC) {
@@ -300,12 +304,12 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
 }
 
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, double floor_rel2, int sort, int debug, int max_inner, hipStream_t s) {
+                         int sort_sweeps, int debug, int max_inner, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
-        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug, max_inner);
+        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, sort_sweeps, debug, max_inner);
     else
-        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, tol, floor_rel2, sort, debug, max_inner);
+        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, sort_sweeps, debug, max_inner);
 }
 
 // ||X||_F^2 of every item: each workgroup reduces one chunk and adds it atomically

@@ -67,6 +67,8 @@ struct JacItem {
     int* rot;       // rotations performed in the current window (device counter)
     int* conv;      // 1 once converged (kernels skip the item)
     double* fro2;   // ||X||_F^2 (device scalar, filled before the sweeps)
+    double tol;     // relative orthogonality target of this matrix
+    double floor_rel2;  // deflation floor relative to ||X||_F^2
 };
 // Columns with ||x||^2 <= JAC_FLOOR(p)^2 ||X||_F^2 are below the rounding floor of the
 // rotations and are treated as zero (frozen): JAC_FLOOR(p) = 4 sqrt(p) eps.
@@ -75,7 +77,7 @@ void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipSt
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         double tol, double floor_rel2, int sort, int debug, int max_inner, hipStream_t s);
+                         int sort_sweeps, int debug, int max_inner, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // converged <- (no rotation in the window); *remaining = items still rotating

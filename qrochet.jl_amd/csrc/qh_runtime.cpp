@@ -207,20 +207,23 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
         for (int k = 0; k < nb / 2; ++k) tiles.push_back(make_int2((int)i, k));
     }
     int* remaining = counters + 2 * items.size();
+    // Relative orthogonality target |x_i^H x_j| <= tol ||x_i|| ||x_j||.  The Gram
+    // entries carry O(sqrt(p)·ε) rounding noise, so the target sits a factor 16
+    // above it; otherwise noise-triggered rotations never stop.  Columns below
+    // 4 sqrt(p) ε ||X||_F are frozen.  Both depend only on the matrix itself, so a
+    // matrix gets the same result alone and inside any batch.
+    for (JacItem& it : items) {
+        const double sp = std::sqrt((double)std::max(it.p, 4));
+        it.tol = 16.0 * sp * DBL_EPSILON;
+        const double fl = 4.0 * sp * DBL_EPSILON;
+        it.floor_rel2 = fl * fl;
+    }
     JacItem* di = upload(c, items);
     int2* dti = upload(c, id_tiles);
     int2* dt = upload(c, tiles);
     int2* dtf = upload(c, frob_tiles);
     launch_set_identity_items(di, dti, (int)id_tiles.size(), c->stream);
     launch_frob2(di, dtf, (int)frob_tiles.size(), c->stream);
-    // Relative orthogonality target |x_i^H x_j| <= tol ||x_i|| ||x_j||.  The Gram
-    // entries carry O(sqrt(p)·ε) rounding noise, so the target sits a factor 16
-    // above it; otherwise noise-triggered rotations never stop.
-    int pmax = 1;
-    for (const JacItem& it : items) pmax = std::max(pmax, it.p);
-    const double tol = 16.0 * std::sqrt((double)std::max(pmax, 4)) * DBL_EPSILON;
-    const double floor_rel = 4.0 * std::sqrt((double)std::max(pmax, 4)) * DBL_EPSILON;
-    const double floor_rel2 = floor_rel * floor_rel;
     // algorithmic work per checked / rotated block pair (for the timing report)
     std::vector<double> gram_f(items.size()), upd_f(items.size()), gram_b(items.size()), upd_b(items.size());
     for (size_t i = 0; i < items.size(); ++i) {
@@ -243,8 +246,8 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
             static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
             for (int r = 0; r < rmax; ++r) {
                 const int g = sweep * rmax + r;
-                launch_jacobi_round(di, dt, (int)tiles.size(), g, block, tol, floor_rel2,
-                                    sweep < c->jac_sort_sweeps ? 1 : 0, kdbg, c->jac_max_inner, c->stream);
+                launch_jacobi_round(di, dt, (int)tiles.size(), g, block, c->jac_sort_sweeps, kdbg, c->jac_max_inner,
+                                    c->stream);
             }
         }
         static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
