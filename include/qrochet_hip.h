@@ -82,6 +82,9 @@ int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_m
 int qh_ctx_reset_timing(qh_context* ctx);
 /* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default. */
 int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);
+/* Device memory of the context's GPU (no reference counterpart: the request loop
+ * sizes its admission budget from total_bytes). */
+int qh_ctx_mem_info(qh_context* ctx, size_t* free_bytes, size_t* total_bytes);
 
 /* ---- MPS objects -------------------------------------------------------- */
 /* Replaces `MPS(arrays)` / `Chain(::State, ::Open, arrays)` (src/Ansatz/Chain.jl:18,64-100)
@@ -118,9 +121,11 @@ int qh_evolve2(qh_context* ctx, qh_mps* mps, int bond, const double* gate, doubl
 /* A brickwork layer: `count` independent 2-site updates, each on its own MPS or on
  * disjoint bonds of a shared one, in one pipeline of batched launches (no
  * reference counterpart as one call: it is `count` calls of evolve! fused).
- * gates: count*d^4 complex.  kept/discarded: count entries or NULL. */
+ * gates: count*d^4 complex.  thresholds / maxdims: per-update truncation (count
+ * entries each, NaN / negative = nothing), or NULL for "nothing" on every update.
+ * kept/discarded: count entries or NULL. */
 int qh_evolve2_batch(qh_context* ctx, int count, qh_mps* const* mps, const int* bonds, const double* gates,
-                     double threshold, int maxdim, unsigned flags, int* kept, double* discarded);
+                     const double* thresholds, const int* maxdims, unsigned flags, int* kept, double* discarded);
 
 /* ---- truncation & canonical forms --------------------------------------- */
 /* `truncate!(ψ, bond; threshold, maxdim)` (src/Ansatz/Chain.jl:390-422). */

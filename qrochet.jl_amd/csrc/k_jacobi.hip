@@ -312,8 +312,11 @@ void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles
         k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, sort_sweeps, debug, max_inner);
 }
 
-// ||X||_F^2 of every item: each workgroup reduces one chunk and adds it atomically
-__global__ __launch_bounds__(256) void k_frob2(const JacItem* __restrict__ items, const int2* __restrict__ tiles) {
+// ||X||_F^2 of every item, deterministic: each workgroup reduces one chunk into
+// partial[tile]; a second pass adds an item's chunks in order.  The sum, and so the
+// deflation floor, does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_frob2(const JacItem* __restrict__ items, const int2* __restrict__ tiles,
+                                               double* __restrict__ partial) {
     __shared__ double sRed[4];
     const int2 tt = tiles[blockIdx.x];
     const JacItem it = items[tt.x];
@@ -326,11 +329,23 @@ __global__ __launch_bounds__(256) void k_frob2(const JacItem* __restrict__ items
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) sRed[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(it.fro2, sRed[0] + sRed[1] + sRed[2] + sRed[3]);
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sRed[0] + sRed[1]) + (sRed[2] + sRed[3]);
 }
-void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
-    if (ntiles <= 0) return;
-    k_frob2<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+// item i owns partial[first[i] .. first[i+1])
+__global__ void k_frob2_sum(const JacItem* __restrict__ items, int count, const double* __restrict__ partial,
+                            const int* __restrict__ first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double s = 0.0;
+    for (int k = first[i]; k < first[i + 1]; ++k) s += partial[k];
+    *items[i].fro2 = s;
+    if (!isfinite(s)) *items[i].conv = -1;  // NaN / Inf input: never rotated, reported by the host
+}
+void launch_frob2(const JacItem* d_items, int count, const int2* d_tiles, int ntiles, double* partial,
+                  const int* first, hipStream_t s) {
+    if (count <= 0) return;
+    if (ntiles > 0) k_frob2<<<ntiles, 256, 0, s>>>(d_items, d_tiles, partial);
+    k_frob2_sum<<<(count + 255) / 256, 256, 0, s>>>(d_items, count, partial, first);
 }
 
 // V <- I for every item (grid: items × column tiles)
@@ -348,22 +363,29 @@ void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int 
     k_set_identity<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 
-// After a window of rounds: an item with no rotation is converged.  `remaining`
-// receives the number of items still rotating.
+// After a window of rounds: an item with no rotation is converged.  remaining[0]
+// receives the number of items still rotating, remaining[1] the number with
+// non-finite entries.
 __global__ void k_conv_update(const JacItem* __restrict__ items, int count, int* remaining) {
-    __shared__ int sRem;
-    if (threadIdx.x == 0) sRem = 0;
+    __shared__ int sRem, sBad;
+    if (threadIdx.x == 0) sRem = sBad = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         const JacItem it = items[i];
-        if (*it.conv == 0) {
+        const int cv = *it.conv;
+        if (cv == 0) {
             if (*it.rot == 0) *it.conv = 1;
             else atomicAdd(&sRem, 1);
+        } else if (cv < 0) {
+            atomicAdd(&sBad, 1);
         }
         *it.rot = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *remaining = sRem;
+    if (threadIdx.x == 0) {
+        remaining[0] = sRem;
+        remaining[1] = sBad;
+    }
 }
 void launch_conv_update(const JacItem* d_items, int count, int* remaining, hipStream_t s) {
     k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining);

@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -24,17 +25,26 @@ thread_local std::string g_err;
 template <class F>
 int guard(qh_context* c, F&& f) {
     try {
-        if (c) QH_HIP(hipSetDevice(c->device));
+        if (c) {
+            QH_HIP(hipSetDevice(c->device));
+            c->call_allocs.clear();
+        }
         f();
-        if (c) c->err.clear();
+        if (c) {
+            c->err.clear();
+            c->call_allocs.clear();
+        }
         return QH_OK;
     } catch (const Error& e) {
+        if (c) release_call_allocs(c);
         (c ? c->err : g_err) = e.what();
         return e.code;
     } catch (const std::bad_alloc&) {
+        if (c) release_call_allocs(c);
         (c ? c->err : g_err) = "host allocation failed";
         return QH_ERR_OOM;
     } catch (const std::exception& e) {
+        if (c) release_call_allocs(c);
         (c ? c->err : g_err) = e.what();
         return QH_ERR_HIP;
     }
@@ -69,16 +79,17 @@ double* dcopy_d(qh_context* c, const double* src, size_t n) {
 void normalize_core(qh_context* c, qh_mps* m, int root);
 
 void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds, const double* gates,
-                  double threshold, int maxdim, unsigned flags, int* kept_out, double* disc_out) {
+                  const double* thresholds, const int* maxdims, unsigned flags, int* kept_out, double* disc_out) {
     if (count <= 0) return;
     if (!ms || !bonds || !gates) throw Error(QH_ERR_ARGUMENT, "null argument");
     const unsigned prec = flags & QH_PREC_MASK;
     if (prec != QH_PREC_FP64) throw Error(QH_ERR_ARGUMENT, "requested contraction precision is not available");
     const bool canonical = flags & QH_CANONICAL;
     const bool renorm = flags & QH_RENORMALIZE;
-    const bool trunc = !std::isnan(threshold) || maxdim >= 0;
-    if (maxdim == 0) throw Error(QH_ERR_ARGUMENT, "maxdim must be positive");
-    const double thr = std::isnan(threshold) ? 1e-16 : threshold;
+    auto thr_of = [&](int i) { return thresholds ? thresholds[i] : std::nan(""); };
+    auto md_of = [&](int i) { return maxdims ? maxdims[i] : -1; };
+    for (int i = 0; i < count; ++i)
+        if (md_of(i) == 0) throw Error(QH_ERR_ARGUMENT, "maxdim must be positive");
     const int d = ms[0] ? ms[0]->d : 0;
     for (int i = 0; i < count; ++i) {
         check_mps(c, ms[i]);
@@ -116,11 +127,14 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
         t.transposed = mr < mc ? 1 : 0;
         if (t.rs) { pl[i] = dalloc<double>(c, chil); launch_pinv(t.rs, chil, 1e-32, pl[i], c->stream); }
         if (t.cs) { pr[i] = dalloc<double>(c, chir); launch_pinv(t.cs, chir, 1e-32, pr[i], c->stream); }
+        const double threshold = thr_of(i);
+        const int maxdim = md_of(i);
+        const bool trunc = !std::isnan(threshold) || maxdim >= 0;
         SvdTask& s = tasks[i];
         s.x = xbuf[i];
         s.mr = mr; s.mc = mc;
         s.truncate = trunc ? 1 : 0;
-        s.threshold = thr;
+        s.threshold = std::isnan(threshold) ? 1e-16 : threshold;
         s.maxdim = maxdim;
         s.renormalize = (canonical && renorm && trunc) ? 1 : 0;
         const int kmax = s.kmax();
@@ -151,8 +165,9 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
         if (kept_out) kept_out[i] = tasks[i].kept;
         if (disc_out) disc_out[i] = tasks[i].discarded;
     }
-    if (renorm && !canonical && trunc)
-        for (int i = 0; i < count; ++i) normalize_core(c, ms[i], bonds[i]);
+    if (renorm && !canonical)
+        for (int i = 0; i < count; ++i)
+            if (tasks[i].truncate) normalize_core(c, ms[i], bonds[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -643,6 +658,11 @@ int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
     });
 }
 
+int qh_ctx_mem_info(qh_context* c, size_t* free_bytes, size_t* total_bytes) {
+    if (!c || !free_bytes || !total_bytes) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { QH_HIP(hipMemGetInfo(free_bytes, total_bytes)); });
+}
+
 int qh_mps_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays,
                   const double* const* lambdas, qh_mps** out) {
     if (!c || !out) return QH_ERR_ARGUMENT;
@@ -675,12 +695,14 @@ int qh_mps_create(qh_context* c, int n, int d, const int* bond_dims, const doubl
                     for (int o = 0; o < d; ++o)
                         st[l + (size_t)cl * (o + (size_t)d * r)] = src[o + (size_t)d * (l + (size_t)cl * r)];
             m->site[i - 1] = dalloc<cplx>(c, ne);
+            keep(c, m->site[i - 1]);
             QH_HIP(hipMemcpyAsync(m->site[i - 1], st.data(), ne * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
         }
         if (lambdas) {
             for (int b = 1; b <= n - 1; ++b) {
                 if (!lambdas[b - 1]) continue;
                 m->lam[b - 1] = dalloc<double>(c, m->chi[b]);
+                keep(c, m->lam[b - 1]);
                 QH_HIP(hipMemcpyAsync(m->lam[b - 1], lambdas[b - 1], m->chi[b] * sizeof(double), hipMemcpyHostToDevice,
                                       c->stream));
             }
@@ -703,11 +725,14 @@ int qh_mps_clone(qh_context* c, const qh_mps* src, qh_mps** out) {
     if (!c || !out) return QH_ERR_ARGUMENT;
     return guard(c, [&] {
         check_mps(c, src);
-        qh_mps* m = new qh_mps(*src);
+        // buffers stay call-owned until the copy is complete: a failure frees them all
+        std::unique_ptr<qh_mps> m(new qh_mps(*src));
         for (int i = 1; i <= m->n; ++i) m->site[i - 1] = dcopy(c, src->site[i - 1], site_elems(src, i));
         for (int b = 1; b <= m->n - 1; ++b)
             m->lam[b - 1] = src->lam[b - 1] ? dcopy_d(c, src->lam[b - 1], src->chi[b]) : nullptr;
-        *out = m;
+        for (auto p : m->site) keep(c, p);
+        for (auto p : m->lam) keep(c, p);
+        *out = m.release();
     });
 }
 
@@ -795,13 +820,13 @@ int qh_evolve1(qh_context* c, qh_mps* m, int site, const double* gate) {
 int qh_evolve2(qh_context* c, qh_mps* m, int bond, const double* gate, double threshold, int maxdim, unsigned flags,
                int* kept, double* discarded) {
     if (!c) return QH_ERR_ARGUMENT;
-    return guard(c, [&] { evolve2_core(c, 1, &m, &bond, gate, threshold, maxdim, flags, kept, discarded); });
+    return guard(c, [&] { evolve2_core(c, 1, &m, &bond, gate, &threshold, &maxdim, flags, kept, discarded); });
 }
 
 int qh_evolve2_batch(qh_context* c, int count, qh_mps* const* ms, const int* bonds, const double* gates,
-                     double threshold, int maxdim, unsigned flags, int* kept, double* discarded) {
+                     const double* thresholds, const int* maxdims, unsigned flags, int* kept, double* discarded) {
     if (!c) return QH_ERR_ARGUMENT;
-    return guard(c, [&] { evolve2_core(c, count, ms, bonds, gates, threshold, maxdim, flags, kept, discarded); });
+    return guard(c, [&] { evolve2_core(c, count, ms, bonds, gates, thresholds, maxdims, flags, kept, discarded); });
 }
 
 int qh_truncate(qh_context* c, qh_mps* m, int bond, double threshold, int maxdim) {

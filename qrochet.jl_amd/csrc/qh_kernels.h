@@ -73,7 +73,8 @@ struct JacItem {
 // Columns with ||x||^2 <= JAC_FLOOR(p)^2 ||X||_F^2 are below the rounding floor of the
 // rotations and are treated as zero (frozen): JAC_FLOOR(p) = 4 sqrt(p) eps.
 // ||X||_F^2 per item (tile map: item, chunk of 64k elements)
-void launch_frob2(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+void launch_frob2(const JacItem* d_items, int count, const int2* d_tiles, int ntiles, double* partial,
+                  const int* first, hipStream_t s);
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,

@@ -42,11 +42,23 @@ void* dalloc_bytes(qh_context* c, size_t bytes) {
     hipError_t e = hipMallocAsync(&p, bytes, c->stream);
     if (e != hipSuccess || p == nullptr)
         throw Error(QH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    c->call_allocs.insert(p);
     return p;
 }
 
 void dfree(qh_context* c, void* p) {
-    if (p) (void)hipFreeAsync(p, c->stream);
+    if (!p) return;
+    c->call_allocs.erase(p);
+    (void)hipFreeAsync(p, c->stream);
+}
+
+void keep(qh_context* c, void* p) {
+    if (p) c->call_allocs.erase(p);
+}
+
+void release_call_allocs(qh_context* c) {
+    for (void* p : c->call_allocs) (void)hipFreeAsync(p, c->stream);
+    c->call_allocs.clear();
 }
 
 void* arena_upload(qh_context* c, const void* src, size_t bytes) {
@@ -186,11 +198,11 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     const int bc = 2 * block;
     std::vector<int2> id_tiles, tiles;
     int rmax = 1;
-    int* counters = dalloc<int>(c, 2 * items.size() + 1);
-    QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 1) * sizeof(int), c->stream));
+    int* counters = dalloc<int>(c, 2 * items.size() + 2);
+    QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 2) * sizeof(int), c->stream));
     double* fro2 = dalloc<double>(c, items.size());
-    QH_HIP(hipMemsetAsync(fro2, 0, items.size() * sizeof(double), c->stream));
     std::vector<int2> frob_tiles;
+    std::vector<int> frob_first(1, 0);
     for (size_t i = 0; i < items.size(); ++i) {
         JacItem& it = items[i];
         int nb = (it.q + block - 1) / block;
@@ -202,6 +214,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
         it.fro2 = fro2 + i;
         const size_t tot = (size_t)it.p * it.q;
         for (size_t k = 0; k * FROB_CHUNK < tot; ++k) frob_tiles.push_back(make_int2((int)i, (int)k));
+        frob_first.push_back((int)frob_tiles.size());
         rmax = std::max(rmax, nb - 1);
         for (int k = 0; k < it.q; ++k) id_tiles.push_back(make_int2((int)i, k));
         for (int k = 0; k < nb / 2; ++k) tiles.push_back(make_int2((int)i, k));
@@ -223,7 +236,8 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     int2* dt = upload(c, tiles);
     int2* dtf = upload(c, frob_tiles);
     launch_set_identity_items(di, dti, (int)id_tiles.size(), c->stream);
-    launch_frob2(di, dtf, (int)frob_tiles.size(), c->stream);
+    double* frob_part = dalloc<double>(c, std::max<size_t>(frob_tiles.size(), 1));
+    launch_frob2(di, (int)items.size(), dtf, (int)frob_tiles.size(), frob_part, upload(c, frob_first), c->stream);
     // algorithmic work per checked / rotated block pair (for the timing report)
     std::vector<double> gram_f(items.size()), upd_f(items.size()), gram_b(items.size()), upd_b(items.size());
     for (size_t i = 0; i < items.size(); ++i) {
@@ -272,9 +286,20 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
         }
         launch_conv_update(di, (int)items.size(), remaining, c->stream);
         c->timers["jacobi_sweep"].launches += 1;
-        int rem = 0;
-        readback(c, &rem, remaining, sizeof(int));
-        if (rem == 0) {
+        int rem[2] = {0, 0};
+        readback(c, rem, remaining, sizeof(rem));
+        if (rem[1] > 0) {
+            std::vector<int> h(2 * items.size());
+            readback(c, h.data(), counters, h.size() * sizeof(int));
+            std::string msg = "non-finite entries in the matrix to decompose:";
+            for (size_t i = 0; i < items.size(); ++i)
+                if (h[2 * i + 1] < 0) msg += " item " + std::to_string(i);
+            dfree(c, counters);
+            dfree(c, fro2);
+            dfree(c, frob_part);
+            throw Error(QH_ERR_NUMERIC, msg);
+        }
+        if (rem[0] == 0) {
             done = true;
             break;
         }
@@ -306,10 +331,12 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
             }
         dfree(c, counters);
         dfree(c, fro2);
+        dfree(c, frob_part);
         throw Error(QH_ERR_NOT_CONVERGED, msg);
     }
     dfree(c, counters);
     dfree(c, fro2);
+    dfree(c, frob_part);
 }
 
 // ---------------------------------------------------------------------------
@@ -580,11 +607,13 @@ size_t site_elems(const qh_mps* m, int site) {
 
 void replace_site(qh_mps* m, int site, cplx* buf) {
     dfree(m->ctx, m->site[site - 1]);
+    keep(m->ctx, buf);
     m->site[site - 1] = buf;
 }
 
 void replace_lam(qh_mps* m, int bond, double* buf) {
     dfree(m->ctx, m->lam[bond - 1]);
+    keep(m->ctx, buf);
     m->lam[bond - 1] = buf;
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <unordered_set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -57,6 +58,9 @@ struct qh_context {
     int jac_sort_sweeps = 2;  // sweeps that order eigenpairs by size (then plain cyclic)
     int jac_precondition = 1; // QR-precondition SVDs with more than one block pair
     int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
+    // device buffers allocated by the entry point running now and not yet owned by
+    // an MPS: freed if that call fails, so an error leaks nothing
+    std::unordered_set<void*> call_allocs;
 };
 
 struct qh_mps {
@@ -77,6 +81,10 @@ T* dalloc(qh_context* c, size_t n) {
     return static_cast<T*>(dalloc_bytes(c, n * sizeof(T)));
 }
 void dfree(qh_context* c, void* p);
+// `p` now belongs to an MPS and outlives the current call
+void keep(qh_context* c, void* p);
+// free what the failed call left behind
+void release_call_allocs(qh_context* c);
 // copy host bytes to the descriptor arena; returns a device pointer valid until the next sync
 void* arena_upload(qh_context* c, const void* src, size_t bytes);
 template <class T>

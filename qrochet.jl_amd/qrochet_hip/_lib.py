@@ -51,6 +51,7 @@ SIGNATURES = {
     "qh_ctx_get_timing": (C.c_int, [_ctx_p, C.c_char_p, _dp, C.POINTER(C.c_int64), _dp, _dp]),
     "qh_ctx_reset_timing": (C.c_int, [_ctx_p]),
     "qh_ctx_set_jacobi": (C.c_int, [_ctx_p, C.c_int, C.c_int]),
+    "qh_ctx_mem_info": (C.c_int, [_ctx_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "qh_mps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_mps_p)]),
     "qh_mps_clone": (C.c_int, [_ctx_p, _mps_p, C.POINTER(_mps_p)]),
     "qh_mps_free": (C.c_int, [_mps_p]),
@@ -63,7 +64,7 @@ SIGNATURES = {
     "qh_evolve2": (C.c_int, [_ctx_p, _mps_p, C.c_int, _dp, C.c_double, C.c_int, C.c_uint, _ip, _dp]),
     "qh_evolve2_batch": (
         C.c_int,
-        [_ctx_p, C.c_int, C.POINTER(_mps_p), _ip, _dp, C.c_double, C.c_int, C.c_uint, _ip, _dp],
+        [_ctx_p, C.c_int, C.POINTER(_mps_p), _ip, _dp, _dp, _ip, C.c_uint, _ip, _dp],
     ),
     "qh_truncate": (C.c_int, [_ctx_p, _mps_p, C.c_int, C.c_double, C.c_int]),
     "qh_contract_between": (C.c_int, [_ctx_p, _mps_p, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -174,6 +174,13 @@ class Context:
     def set_jacobi(self, block=0, max_sweeps=0):
         self.check(L.load().qh_ctx_set_jacobi(self.handle, block, max_sweeps))
 
+    def mem_info(self):
+        """(free, total) bytes of the device."""
+        f = C.c_size_t()
+        t = C.c_size_t()
+        self.check(L.load().qh_ctx_mem_info(self.handle, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
 
 _default = {}
 
@@ -447,7 +454,9 @@ def simple_update_(psi: Chain, gate: Dense, threshold=None, maxdim=None, renorma
 def evolve_layer_(updates, threshold=None, maxdim=None, iscanonical=False, renormalize=False, precision="fp64",
                   context: Context | None = None):
     """A brickwork layer: ``updates`` is a list of (Chain, 2-site Dense) on disjoint
-    sites, applied in one batched pipeline (`qh_evolve2_batch`).  Returns (kept, discarded)."""
+    sites, applied in one batched pipeline (`qh_evolve2_batch`).  ``threshold`` and
+    ``maxdim`` are either one value for every update or a list with one entry per
+    update (None = nothing).  Returns (kept, discarded)."""
     if not updates:
         return [], []
     ctx = context or updates[0][0].ctx
@@ -463,11 +472,18 @@ def evolve_layer_(updates, threshold=None, maxdim=None, iscanonical=False, renor
         bonds[i] = lanes[0]
         gates.append(g.canonical())
     gbuf = np.ascontiguousarray(np.concatenate(gates))
-    thr, md = _trunc_args(threshold, maxdim)
+    ths = threshold if isinstance(threshold, (list, tuple, np.ndarray)) else [threshold] * n
+    mds = maxdim if isinstance(maxdim, (list, tuple, np.ndarray)) else [maxdim] * n
+    if len(ths) != n or len(mds) != n:
+        raise ArgumentError("one threshold / maxdim per update")
+    tv = (C.c_double * n)()
+    mv = (C.c_int * n)()
+    for i in range(n):
+        tv[i], mv[i] = _trunc_args(ths[i], mds[i])
     flags = _PREC[precision] | (L.QH_CANONICAL if iscanonical else 0) | (L.QH_RENORMALIZE if renormalize else 0)
     kept = (C.c_int * n)()
     disc = (C.c_double * n)()
-    ctx.check(L.load().qh_evolve2_batch(ctx.handle, n, handles, bonds, gbuf.ctypes.data_as(L._dp), thr, md, flags,
+    ctx.check(L.load().qh_evolve2_batch(ctx.handle, n, handles, bonds, gbuf.ctypes.data_as(L._dp), tv, mv, flags,
                                         kept, disc))
     return list(kept), list(disc)
 
