@@ -14,7 +14,13 @@ Printed JSON: updates/s over the timed steps, plus
   * the plain FP64 per-call path (one evolve! per bond) on a bounded sample of
     saturated (χ=256) layers, for the batched/plain ratio,
   * the CPU oracle (numpy/LAPACK zgesdd restatement) on a bounded sample of the
-    same saturated layers (`cpu_baseline`).
+    same saturated layers (`cpu_baseline`),
+  * the request loop (`serving`): Poisson arrivals at a fixed QPS of brickwork
+    Haar-circuit requests, p50/p99 latency and updates/s.  The default run uses
+    a bounded sample of the serving distribution (n ~ U{8..64}, χmax
+    log-uniform in {32..256}, depth ~ U{1..40}); `--serving full
+    --serving-only` runs the full one (n ~ U{8..128}, χmax log-uniform in
+    {32..1024}) and prints its own line.
 """
 from __future__ import annotations
 
@@ -52,6 +58,45 @@ def run_workload(qh, arrays, layers, trunc, precision="fp64", stop_layer=None):
     return psi, nup
 
 
+SERVING_SAMPLE = dict(n_range=(8, 64), chi_choices=(32, 64, 128, 256), depth_range=(1, 40))
+SERVING_FULL = dict(n_range=(8, 128), chi_choices=(32, 64, 128, 256, 512, 1024), depth_range=(1, 40))
+
+
+def serving_leg(kind, requests, qps, seed=2025):
+    """Run the request loop on `requests` Poisson arrivals at `qps` (real time)."""
+    from qrochet_hip import serving as sv
+
+    dist = SERVING_FULL if kind == "full" else SERVING_SAMPLE
+    reqs = sv.make_requests(seed, requests, qps, **dist)
+    srv = sv.Server()
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True,
+                  log=lambda r: log(f"request {r.rid} {r.status} latency {r.latency:.2f} s ({r.updates} updates)"))
+    wall = time.perf_counter() - t0
+    ok = [r for r in res if r.status == "ok"]
+    updates = sum(r.updates for r in ok)
+    out = {
+        "workload": (f"{'full' if kind == 'full' else 'sample of the'} serving distribution: Poisson arrivals, "
+                     f"n~U{{{dist['n_range'][0]}..{dist['n_range'][1]}}}, d=2, chimax log-uniform in "
+                     f"{{{','.join(map(str, dist['chi_choices']))}}}, depth~U{{{dist['depth_range'][0]}.."
+                     f"{dist['depth_range'][1]}}}, Haar brickwork from |0..0>, cutoff 1e-12, observables Z_i and "
+                     f"~n random Z_iZ_j (|i-j|<=4), seed {seed}"),
+        "qps": qps,
+        "requests": requests,
+        "lanes_chimax": list(srv.lane_bounds),
+    }
+    out.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in sv.latency_summary(res).items()})
+    out.update({
+        "refused": srv.stats["refused"],
+        "failed": sum(r.status not in ("ok", "admission") for r in res),
+        "updates": updates,
+        "wall_s": round(wall, 3),
+        "updates_per_s": round(updates / wall, 2),
+        "iterations": srv.stats["iterations"],
+    })
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +107,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serving", choices=("sample", "full", "off"), default="sample")
+    ap.add_argument("--serving-only", action="store_true", help="run only the request loop, print its own line")
+    ap.add_argument("--serving-requests", type=int, default=None)
+    ap.add_argument("--qps", type=float, default=None)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy (known bytes) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is not built.",
@@ -69,6 +120,35 @@ def main():
         sys.exit(2)
 
     import qrochet_hip as qh
+
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: 1 GiB read, 1 GiB written
+        torch.cuda.synchronize()
+        log("calibration copy: 3 launches, each 1 GiB read + 1 GiB written (torch elementwise mul by 1)")
+        del src, dst
+
+    if args.serving != "off":
+        full = args.serving == "full"
+        n_req = args.serving_requests or (40 if full else 48)
+        qps = args.qps or (0.1 if full else 4.0)
+    if args.serving_only:
+        if args.serving == "off":
+            sys.exit("--serving-only needs --serving sample|full")
+        sv_out = serving_leg(args.serving, n_req, qps)
+        print(json.dumps({
+            "metric": "p99 latency of brickwork-circuit requests at fixed QPS (request loop)",
+            "value": sv_out.get("p99_s"), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+            "config": {"workload": sv_out["workload"], "qps": qps, "requests": n_req},
+            "serving": sv_out,
+        }))
+        return
+
     from oracle import workloads as wl
 
     arrays, layers, trunc, _ = wl.config2(steps=args.trotter)
@@ -203,6 +283,9 @@ def main():
         "jacobi_sweeps": timers["jacobi_sweep"]["launches"],
     }
     out.update(extra)
+    if args.serving != "off":
+        log(f"serving leg: {n_req} requests at {qps} QPS")
+        out["serving"] = serving_leg(args.serving, n_req, qps)
     print(json.dumps(out))
 
 

@@ -11,11 +11,12 @@ gate on one ``Chain`` (src/Ansatz/Chain.jl:543-661) and ``expect`` at the end
   footprint (state at the χmax profile plus one layer of SVD workspace) fits the
   memory budget next to the requests already running; one whose footprint alone
   exceeds the budget is refused with status ``"admission"``;
-* every iteration advances each running request by one brickwork layer, all of
-  them in ONE batched update pipeline (`qh_evolve2_batch`), so the work of
-  requests that arrived at different times overlaps on the device; a request
-  that reaches its depth is measured (`qh_expect_many`) and released at once,
-  and its memory admits the next one;
+* admitted requests join a lane by χmax; every iteration of a lane advances each
+  of its running requests by one brickwork layer, all of them in ONE batched
+  update pipeline (`qh_evolve2_batch`), so the work of requests that arrived at
+  different times overlaps on the device; lanes run concurrently on their own
+  streams; a request that reaches its depth is measured (`qh_expect_many`) and
+  released at once, and its memory admits the next one;
 * a failure inside a batch (non-finite tensor, non-convergence) is isolated by
   re-running that iteration request by request: the batched call writes no state
   before every decomposition of the batch has succeeded, so the failed request
@@ -30,6 +31,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import threading
 import time
 
 import numpy as np
@@ -162,11 +164,12 @@ _STATUS = {L.QH_ERR_NUMERIC: "numeric", L.QH_ERR_NOT_CONVERGED: "not_converged",
 
 
 class HipEngine:
-    """What the loop runs on: states in device memory, one `qh_evolve2_batch` per
-    layer of the whole batch, `qh_expect_many` at completion."""
+    """What a lane runs on: its own context (one HIP stream), states in device
+    memory, one `qh_evolve2_batch` per layer of the lane's batch, `qh_expect_many`
+    at completion."""
 
     def __init__(self, context: Context | None = None):
-        self.ctx = context or default_context()
+        self.ctx = context or Context()
 
     def total_memory(self) -> int:
         return self.ctx.mem_info()[1]
@@ -196,101 +199,70 @@ class _Running:
         self.layer = 0
 
 
-class Server:
-    """The request loop.  ``hbm_budget`` (bytes) bounds the summed projected
-    footprint of running requests (default: 85 % of the device memory);
-    ``max_active`` optionally bounds their number."""
+class _Lane(threading.Thread):
+    """One batching lane: a host thread driving its own engine (own HIP stream),
+    advancing its running requests one brickwork layer per batched call."""
 
-    def __init__(self, engine=None, hbm_budget: int | None = None, max_active: int | None = None):
-        self.engine = engine or HipEngine()
-        if hbm_budget is None:
-            hbm_budget = int(0.85 * self.engine.total_memory())
-        self.budget = int(hbm_budget)
-        self.max_active = max_active
-        self.stats = collections.Counter()
+    def __init__(self, server: "Server", index: int, engine):
+        super().__init__(name=f"qh-lane-{index}", daemon=True)
+        self.srv, self.index, self.engine = server, index, engine
+        self.inbox: collections.deque = collections.deque()
+        self.active: list[_Running] = []
+        self.error: BaseException | None = None
 
-    def run(self, requests, realtime: bool = True, log=None) -> list[Result]:
-        """Serve ``requests`` (arrival times relative to the call).  With ``realtime``
-        False an idle server jumps to the next arrival instead of sleeping (results
-        are the same; latencies are not meaningful)."""
-        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        waiting = collections.deque()
-        active: list[_Running] = []
-        results: dict[int, Result] = {}
-        used = 0
-        t0 = time.perf_counter()
-        skew = 0.0  # simulated-time offset when not in real time
+    def run(self):
+        try:
+            self._loop()
+        except BaseException as e:  # reported by Server.run
+            self.error = e
+            with self.srv.cv:
+                self.srv.cv.notify_all()
 
-        def now():
-            return time.perf_counter() - t0 + skew
-
-        while pending or waiting or active:
-            t = now()
-            while pending and pending[0].arrival <= t:
-                waiting.append(pending.popleft())
-            # admission, first come first served
-            while waiting:
-                r = waiting[0]
-                fp = footprint(r.n, r.d, r.chimax)
-                if fp > self.budget:
-                    waiting.popleft()
-                    results[r.rid] = Result(r.rid, "admission", f"projected footprint {fp} B exceeds the budget "
-                                            f"{self.budget} B", arrival=r.arrival, finished=now())
-                    self.stats["refused"] += 1
-                    continue
-                if used + fp > self.budget or (self.max_active is not None and len(active) >= self.max_active):
-                    break
-                waiting.popleft()
-                res = Result(r.rid, "ok", arrival=r.arrival, admitted=now())
+    def _loop(self):
+        srv = self.srv
+        while True:
+            with srv.cv:
+                while not self.inbox and not self.active and not srv.closing:
+                    srv.cv.wait()
+                if srv.closing and not self.inbox and not self.active:
+                    return
+                new = list(self.inbox)
+                self.inbox.clear()
+            for r, fp, res in new:
                 try:
                     psi = self.engine.create(r)
                 except QHError as e:
-                    res.status, res.message, res.finished = _STATUS.get(e.code, "error"), str(e), now()
-                    results[r.rid] = res
+                    res.status, res.message = _STATUS.get(e.code, "error"), str(e)
+                    srv._finish(r, fp, res)
                     continue
-                used += fp
-                active.append(_Running(r, psi, fp, res))
-                self.stats["admitted"] += 1
-            if not active:
-                if pending:
-                    wait = pending[0].arrival - now()
-                    if wait > 0:
-                        if realtime:
-                            time.sleep(wait)
-                        else:
-                            skew += wait
+                self.active.append(_Running(r, psi, fp, res))
+            if not self.active:
                 continue
-            # one brickwork layer of every running request, in one batched pipeline
-            self._step(active)
-            self.stats["iterations"] += 1
-            done = [a for a in active if a.layer >= a.req.depth or a.res.status != "ok"]
+            self._step()
+            srv._count("iterations")
+            done = [a for a in self.active if a.layer >= a.req.depth or a.res.status != "ok"]
             for a in done:
                 if a.res.status == "ok":
                     try:
                         a.res.values, a.res.bond_dims = self.engine.measure(a.psi, a.req)
                     except QHError as e:
                         a.res.status, a.res.message = _STATUS.get(e.code, "error"), str(e)
-                a.res.finished = now()
-                results[a.req.rid] = a.res
                 a.psi = None  # releases the state
-                used -= a.fp
-                active.remove(a)
-                if log:
-                    log(a.res)
-        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+                self.active.remove(a)
+                srv._finish(a.req, a.fp, a.res)
 
     @staticmethod
     def _work(a: _Running):
         return (a.psi, a.req.gates(a.layer), a.req.threshold, a.req.chimax)
 
-    def _step(self, active: list[_Running]):
-        work = [self._work(a) for a in active]
+    def _step(self):
+        work = [self._work(a) for a in self.active]
         try:
             self.engine.layer(work)
         except QHError:
             # isolate the failure: no state was written, so rerun request by request
-            self.stats["isolations"] += 1
-            for a, w in zip(active, work):
+            self.srv._count("isolations")
+            for a, w in zip(self.active, work):
                 try:
                     self.engine.layer([w])
                 except QHError as e:
@@ -299,9 +271,122 @@ class Server:
                 a.layer += 1
                 a.res.updates += len(w[1])
             return
-        for a, w in zip(active, work):
+        for a, w in zip(self.active, work):
             a.layer += 1
             a.res.updates += len(w[1])
+
+
+class Server:
+    """The request loop.
+
+    ``lanes``: χmax bounds of the batching lanes; a request joins the first lane
+    whose bound is >= its χmax (the last lane takes the rest).  Each lane batches
+    only requests of similar size, because a batched call lasts as long as its
+    largest decomposition, and runs on its own stream from its own host thread, so
+    lanes share the GPU's compute units concurrently: a χ=32 request is not held
+    behind the layer of a χ=1024 one.  ``engine_factory()`` makes one engine per
+    lane (default: `HipEngine`, a fresh context each).
+
+    ``hbm_budget`` (bytes) bounds the summed projected footprint of admitted
+    requests (default: 85 % of the device memory); ``max_active`` optionally
+    bounds their number.  Admission is first come, first served."""
+
+    def __init__(self, engine_factory=None, hbm_budget: int | None = None, max_active: int | None = None,
+                 lanes=(64, 256, 1024)):
+        self.engine_factory = engine_factory or HipEngine
+        self.lane_bounds = tuple(lanes) if lanes else (1 << 30,)
+        self._hbm_budget = hbm_budget
+        self.budget = None
+        self.max_active = max_active
+        self.stats = collections.Counter()
+        self.cv = threading.Condition()
+        self.closing = False
+
+    def lane_of(self, req: Request) -> int:
+        for i, b in enumerate(self.lane_bounds):
+            if req.chimax <= b:
+                return i
+        return len(self.lane_bounds) - 1
+
+    def _count(self, key, k=1):
+        with self.cv:
+            self.stats[key] += k
+
+    def _finish(self, req, fp, res):
+        with self.cv:
+            res.finished = self._now()
+            self._results[req.rid] = res
+            self._used -= fp
+            self._inflight -= 1
+            self.cv.notify_all()
+        if self._log:
+            self._log(res)
+
+    def run(self, requests, realtime: bool = True, log=None) -> list[Result]:
+        """Serve ``requests`` (arrival times in seconds after the call).  With
+        ``realtime`` False every request is taken as arrived at once, in arrival
+        order (values are the same; latencies are then not meaningful)."""
+        engines = [self.engine_factory() for _ in self.lane_bounds]
+        if self.budget is None:
+            self.budget = int(self._hbm_budget if self._hbm_budget is not None
+                              else 0.85 * engines[0].total_memory())
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        waiting: collections.deque = collections.deque()
+        self._results: dict[int, Result] = {}
+        self._used = 0
+        self._inflight = 0
+        self._log = log
+        self.closing = False
+        t0 = time.perf_counter()
+        self._now = lambda: time.perf_counter() - t0
+        lanes = [_Lane(self, i, e) for i, e in enumerate(engines)]
+        for ln in lanes:
+            ln.start()
+        try:
+            with self.cv:
+                while pending or waiting or self._inflight:
+                    for ln in lanes:
+                        if ln.error is not None:
+                            raise ln.error
+                    t = self._now()
+                    while pending and (not realtime or pending[0].arrival <= t):
+                        waiting.append(pending.popleft())
+                    while waiting:
+                        r = waiting[0]
+                        fp = footprint(r.n, r.d, r.chimax)
+                        if fp > self.budget:
+                            waiting.popleft()
+                            self._results[r.rid] = Result(
+                                r.rid, "admission", f"projected footprint {fp} B exceeds the budget {self.budget} B",
+                                arrival=r.arrival, finished=self._now())
+                            self.stats["refused"] += 1
+                            continue
+                        if self._used + fp > self.budget or (
+                                self.max_active is not None and self._inflight >= self.max_active):
+                            break
+                        waiting.popleft()
+                        self._used += fp
+                        self._inflight += 1
+                        self.stats["admitted"] += 1
+                        res = Result(r.rid, "ok", arrival=r.arrival if realtime else 0.0, admitted=self._now())
+                        lanes[self.lane_of(r)].inbox.append((r, fp, res))
+                        self.cv.notify_all()
+                    if not (pending or waiting or self._inflight):
+                        break
+                    timeout = None
+                    if pending and realtime:
+                        timeout = max(0.0, pending[0].arrival - self._now())
+                    self.cv.wait(timeout)
+        finally:
+            with self.cv:
+                self.closing = True
+                self.cv.notify_all()
+            for ln in lanes:
+                ln.join()
+        for ln in lanes:
+            if ln.error is not None:
+                raise ln.error
+        return [self._results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
 
 
 def latency_summary(results: list[Result]) -> dict:

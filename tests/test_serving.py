@@ -87,10 +87,10 @@ def test_request_generator():
 
 def test_alone_equals_mix():
     reqs = small_requests()
-    mix = sv.Server(OracleEngine()).run(reqs, realtime=False)
+    mix = sv.Server(OracleEngine).run(reqs, realtime=False)
     assert all(r.status == "ok" for r in mix)
     for r, m in zip(reqs, mix):
-        alone = sv.Server(OracleEngine()).run([r], realtime=False)[0]
+        alone = sv.Server(OracleEngine).run([r], realtime=False)[0]
         assert alone.bond_dims == m.bond_dims
         assert np.array_equal(alone.values, m.values)
         assert m.updates == r.updates()
@@ -101,11 +101,11 @@ def test_admission_refuses_oversized_and_serves_the_rest():
     budget = max(sv.footprint(r.n, r.d, r.chimax) for r in reqs) * 2
     big = sv.Request(rid=99, n=64, chimax=256, depth=2, seed=1, arrival=0.0)
     assert sv.footprint(big.n, big.d, big.chimax) > budget
-    srv = sv.Server(OracleEngine(), hbm_budget=budget)
+    srv = sv.Server(OracleEngine, hbm_budget=budget)
     res = srv.run(reqs + [big], realtime=False)
     by = {r.rid: r for r in res}
     assert by[99].status == "admission" and by[99].values is None
-    ref = sv.Server(OracleEngine()).run(reqs, realtime=False)
+    ref = sv.Server(OracleEngine).run(reqs, realtime=False)
     for r in ref:
         assert by[r.rid].status == "ok"
         assert np.array_equal(by[r.rid].values, r.values)
@@ -118,13 +118,13 @@ def test_budget_bounds_concurrency():
         r.arrival = 0.0
     fp = {r.rid: sv.footprint(r.n, r.d, r.chimax) for r in reqs}
     budget = max(fp.values())
-    res = sv.Server(OracleEngine(), hbm_budget=budget).run(reqs, realtime=False)
+    res = sv.Server(OracleEngine, hbm_budget=budget).run(reqs, realtime=False)
     # at every admission the running requests' projected footprints fit the budget
     for r in res:
         live = [q for q in res if q.admitted <= r.admitted < q.finished]
         assert sum(fp[q.rid] for q in live) <= budget
     # one slot: strictly one request after another, same values
-    res2 = sv.Server(OracleEngine(), max_active=1).run(reqs, realtime=False)
+    res2 = sv.Server(OracleEngine, max_active=1).run(reqs, realtime=False)
     spans = sorted((r.admitted, r.finished) for r in res2)
     assert all(spans[k][1] <= spans[k + 1][0] for k in range(len(spans) - 1))
     for a, b in zip(res, res2):
@@ -136,13 +136,12 @@ def test_failed_request_is_isolated():
     for r in reqs:
         r.arrival = 0.0
     bad = sv.Request(rid=50, n=7, chimax=8, depth=4, seed=11, arrival=0.0, poison_layer=1)
-    eng = OracleEngine()
-    srv = sv.Server(eng)
+    srv = sv.Server(OracleEngine)
     res = {r.rid: r for r in srv.run(reqs + [bad], realtime=False)}
     assert res[50].status == "numeric" and "non-finite" in res[50].message
     assert res[50].updates == len(sv.layer_bonds(7, 0))  # layer 0 done, layer 1 failed
     assert srv.stats["isolations"] == 1
-    ref = {r.rid: r for r in sv.Server(OracleEngine()).run(reqs, realtime=False)}
+    ref = {r.rid: r for r in sv.Server(OracleEngine).run(reqs, realtime=False)}
     for rid, r in ref.items():
         assert res[rid].status == "ok"
         assert np.array_equal(res[rid].values, r.values)
@@ -156,3 +155,26 @@ def test_latency_summary():
     assert s["p50_s"] == pytest.approx(50.5)
     assert s["p99_s"] == pytest.approx(99.01)
     assert s["max_s"] == 100.0
+
+
+def test_lanes_split_by_chimax_and_share_nothing():
+    srv = sv.Server(OracleEngine, lanes=(64, 256, 1024))
+    mk = lambda chi: sv.Request(rid=0, n=8, chimax=chi, depth=1, seed=0)  # noqa: E731
+    assert [srv.lane_of(mk(c)) for c in (4, 64, 65, 256, 512, 1024, 4096)] == [0, 0, 1, 1, 2, 2, 2]
+    # the same requests through one lane or three give the same values
+    reqs = sv.make_requests(5, 6, qps=50.0, n_range=(6, 8), chi_choices=(2, 4, 8), depth_range=(2, 4))
+    one = sv.Server(OracleEngine, lanes=None).run(reqs, realtime=False)
+    three = sv.Server(OracleEngine, lanes=(2, 4, 8)).run(reqs, realtime=False)
+    for a, b in zip(one, three):
+        assert a.status == b.status == "ok"
+        assert np.array_equal(a.values, b.values)
+
+
+class _BrokenEngine(OracleEngine):
+    def layer(self, work):
+        raise RuntimeError("engine failure outside the status-code contract")
+
+
+def test_lane_crash_propagates():
+    with pytest.raises(RuntimeError, match="outside the status-code contract"):
+        sv.Server(_BrokenEngine).run(small_requests(count=2), realtime=False)
