@@ -22,6 +22,51 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
     else { a = (round + k) % m; b = (round - k + m) % m; }
 }
 
+// Rotation of the Hermitian 2×2 problem [[g_aa, g_ab], [conj(g_ab), g_bb]]:
+// J = [[c, s e], [-s conj(e), c]] with e = g_ab/|g_ab| and t = s/c the smaller
+// root of t^2 + 2 zeta t - 1 = 0, zeta = (g_bb - g_aa)/(2|g_ab|) (Rutishauser's
+// formulation); naa / nbb are the new diagonal.  No rotation (s = 0) when the
+// pair is already orthogonal, |g_ab|^2 <= tol2·g_aa·g_bb, or a column is below
+// the deflation floor.
+struct JRot {
+    double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = 0.0, nbb = 0.0;
+};
+__device__ __forceinline__ bool needs_rotation(double gaa, double gbb, double2 gab, double tol2, double gfloor) {
+    return gaa > gfloor && gbb > gfloor && cabs2(gab) > tol2 * gaa * gbb;
+}
+__device__ __forceinline__ JRot jacobi_rot(double gaa, double gbb, double2 gab, double tol2, double gfloor) {
+    JRot r;
+    r.naa = gaa;
+    r.nbb = gbb;
+    if (needs_rotation(gaa, gbb, gab, tol2, gfloor)) {
+        const double mag = sqrt(cabs2(gab));
+        const double rmag = 1.0 / mag;
+        const double zeta = (gbb - gaa) * (0.5 * rmag);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        r.c = 1.0 / sqrt(1.0 + t * t);
+        r.s = r.c * t;
+        r.ex = gab.x * rmag;
+        r.ey = gab.y * rmag;
+        r.naa = gaa - t * mag;
+        r.nbb = gbb + t * mag;
+    }
+    return r;
+}
+// (x_a, x_b) <- (x_a, x_b) J
+__device__ __forceinline__ void rot_cols(double2& xa, double2& xb, const JRot& j) {
+    const double2 ev = make_double2(j.ex, j.ey);
+    const double2 na = csub(cscale(xa, j.c), cscale(cmul(cconj(ev), xb), j.s));
+    xb = cadd(cscale(cmul(ev, xa), j.s), cscale(xb, j.c));
+    xa = na;
+}
+// (y_a; y_b) <- J^H (y_a; y_b)
+__device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j) {
+    const double2 ev = make_double2(j.ex, j.ey);
+    const double2 na = csub(cscale(ya, j.c), cscale(cmul(ev, yb), j.s));
+    yb = cadd(cscale(cmul(cconj(ev), ya), j.s), cscale(yb, j.c));
+    ya = na;
+}
+
 template <int BC>
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
                                                       const int2* __restrict__ tiles, int round, int sort_sweeps,
@@ -34,8 +79,6 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     __shared__ double2 sG[BC][LP];
     __shared__ double2 sQ[BC][LP];
     __shared__ double2 sX[BC][CH + 1];
-    __shared__ double sRot[NP][6];
-    __shared__ int sPair[NP][2];
     __shared__ int sCol[BC];
     __shared__ int sPerm[BC];
     __shared__ int sFlag;
@@ -47,6 +90,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     const int R = it.nb - 1;
     const int r = round % R;
     const double tol = it.tol;
+    const double tol2 = tol * tol;
     const double floor_rel2 = it.floor_rel2;
     // eigenpair sorting during this matrix's own first sweeps (independent of the batch)
     const int sort = round < sort_sweeps * R;
@@ -139,7 +183,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         if (i < j) {
             const double gii = sG[i][i].x, gjj = sG[j][j].x;
             const double2 g = sG[i][j];
-            if (gii > gfloor && gjj > gfloor && sqrt(cabs2(g)) > tol * sqrt(gii) * sqrt(gjj)) {
+            if (needs_rotation(gii, gjj, g, tol2, gfloor)) {
                 sFlag = 1;
                 if (debug)
                     printf("[jr] round %d pair(%d,%d) cols %d %d cos/tol %.6f gii %.17e gjj %.17e\n", round, bi, bj,
@@ -151,6 +195,11 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     if (!sFlag) return;
 
     // ---------------- EVD of G by cyclic Jacobi ----------------
+    // A round applies NP disjoint rotations.  Thread (pb, qb) owns the 2×2 block of G
+    // at (row pair pb, column pair qb) and the 2×2 block of Q at rows {2pb, 2pb+1} ×
+    // column pair qb.  It derives both rotations from the round's starting G, forms
+    // J_pb^H (G_blk J_qb) and Q_blk J_qb in registers and writes them back after one
+    // barrier: two barriers per round and no serial phase.
     for (int e = tid; e < BC * BC; e += 256) {
         const int i = e / BC, j = e % BC;
         sQ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
@@ -159,70 +208,59 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     // one block pair per matrix: solve the block exactly; otherwise an approximate
     // diagonaliser suffices (the outer sweeps re-check every pair on fresh Grams)
     const int inner_cap = (it.nb == 2) ? 12 : max_inner;
+    static_assert((NP * NP) % 64 == 0, "block owners must fill whole waves");
+    const bool owner = tid < NP * NP;
+    const int pb = tid / NP, qb = tid % NP;
 #pragma unroll 1
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         if (tid == 0) sFlag = 0;
         __syncthreads();
 #pragma unroll 1
         for (int rnd = 0; rnd < BC - 1; ++rnd) {
-            if (tid < NP) {
-                int a, b;
-                rr_pair(rnd, tid, BC, a, b);
-                const double gaa = sG[a][a].x, gbb = sG[b][b].x;
-                const double2 gab = sG[a][b];
-                const double mag = sqrt(cabs2(gab));
-                double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = gaa, nbb = gbb;
-                if (gaa > gfloor && gbb > gfloor && mag > tol * sqrt(gaa) * sqrt(gbb)) {
-                    const double zeta = (gbb - gaa) / (2.0 * mag);
-                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    s = c * t;
-                    ex = gab.x / mag;
-                    ey = gab.y / mag;
-                    naa = gaa - t * mag;
-                    nbb = gbb + t * mag;
-                    sFlag = 1;
+            int ap = 0, bp = 0, aq = 0, bq = 0;
+            double2 g00, g01, g10, g11, q00, q01, q10, q11;
+            JRot jp, jq;
+            if (owner) {
+                rr_pair(rnd, pb, BC, ap, bp);
+                rr_pair(rnd, qb, BC, aq, bq);
+                // each lane solves its column pair; the row pair's rotation comes from
+                // the lane of the same wave that owns diagonal block (pb, pb)
+                jq = jacobi_rot(sG[aq][aq].x, sG[bq][bq].x, sG[aq][bq], tol2, gfloor);
+                // (whole waves are owners or not, so every lane of the wave shuffles)
+                const int src = (pb % (64 / NP)) * NP + pb;
+                jp.c = __shfl(jq.c, src, 64);
+                jp.s = __shfl(jq.s, src, 64);
+                jp.ex = __shfl(jq.ex, src, 64);
+                jp.ey = __shfl(jq.ey, src, 64);
+                if (pb == qb) jp = jq;
+                g00 = sG[ap][aq]; g01 = sG[ap][bq]; g10 = sG[bp][aq]; g11 = sG[bp][bq];
+                q00 = sQ[2 * pb][aq]; q01 = sQ[2 * pb][bq]; q10 = sQ[2 * pb + 1][aq]; q11 = sQ[2 * pb + 1][bq];
+                if (jq.s != 0.0) {  // columns:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
+                    rot_cols(g00, g01, jq);
+                    rot_cols(g10, g11, jq);
+                    rot_cols(q00, q01, jq);
+                    rot_cols(q10, q11, jq);
                 }
-                sRot[tid][0] = c; sRot[tid][1] = s; sRot[tid][2] = ex; sRot[tid][3] = ey;
-                sRot[tid][4] = naa; sRot[tid][5] = nbb;
-                sPair[tid][0] = a; sPair[tid][1] = b;
+                if (jp.s != 0.0) {
+                    if (pb == qb) {  // the diagonal block becomes diag(λ_a, λ_b)
+                        g00 = make_double2(jp.naa, 0.0);
+                        g11 = make_double2(jp.nbb, 0.0);
+                        g01 = cz();
+                        g10 = cz();
+                    } else {  // rows:  y_a' = c y_a - s e y_b ;  y_b' = s conj(e) y_a + c y_b
+                        rot_rows(g00, g10, jp);
+                        rot_rows(g01, g11, jp);
+                    }
+                }
+                if (pb == qb && jp.s != 0.0) sFlag = 1;
             }
             __syncthreads();
-            // columns of G and Q:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
-#pragma unroll 1
-            for (int e = tid; e < 2 * NP * BC; e += 256) {
-                const int mat = e / (NP * BC), rem = e % (NP * BC);
-                const int kp = rem / BC, row = rem % BC;
-                const double s = sRot[kp][1];
-                if (s == 0.0) continue;
-                const double c = sRot[kp][0];
-                const double2 ev = make_double2(sRot[kp][2], sRot[kp][3]);
-                const int a = sPair[kp][0], b = sPair[kp][1];
-                double2 (*M)[LP] = mat ? sQ : sG;
-                const double2 xa = M[row][a], xb = M[row][b];
-                M[row][a] = csub(cscale(xa, c), cscale(cmul(cconj(ev), xb), s));
-                M[row][b] = cadd(cscale(cmul(ev, xa), s), cscale(xb, c));
-            }
-            __syncthreads();
-            // rows of G:  y_a' = c y_a - s e y_b ;  y_b' = s conj(e) y_a + c y_b
-#pragma unroll 1
-            for (int e = tid; e < NP * BC; e += 256) {
-                const int kp = e / BC, col = e % BC;
-                const double s = sRot[kp][1];
-                if (s == 0.0) continue;
-                const double c = sRot[kp][0];
-                const double2 ev = make_double2(sRot[kp][2], sRot[kp][3]);
-                const int a = sPair[kp][0], b = sPair[kp][1];
-                if (col == a) {
-                    sG[a][a] = make_double2(sRot[kp][4], 0.0);
-                    sG[b][a] = cz();
-                } else if (col == b) {
-                    sG[a][b] = cz();
-                    sG[b][b] = make_double2(sRot[kp][5], 0.0);
-                } else {
-                    const double2 ya = sG[a][col], yb = sG[b][col];
-                    sG[a][col] = csub(cscale(ya, c), cscale(cmul(ev, yb), s));
-                    sG[b][col] = cadd(cscale(cmul(cconj(ev), ya), s), cscale(yb, c));
+            if (owner) {
+                if (jp.s != 0.0 || jq.s != 0.0) {
+                    sG[ap][aq] = g00; sG[ap][bq] = g01; sG[bp][aq] = g10; sG[bp][bq] = g11;
+                }
+                if (jq.s != 0.0) {
+                    sQ[2 * pb][aq] = q00; sQ[2 * pb][bq] = q01; sQ[2 * pb + 1][aq] = q10; sQ[2 * pb + 1][bq] = q11;
                 }
             }
             __syncthreads();

@@ -17,18 +17,50 @@ constexpr int QR_TC = 16;  // target columns per apply workgroup
 constexpr int QR_CH = 32;  // rows per LDS chunk
 
 // block-wide sum of K complex values per thread (256 threads); result in out[K]
+__device__ __forceinline__ double2 shfl_xor2(double2 a, int off) {
+    return make_double2(__shfl_xor(a.x, off, 64), __shfl_xor(a.y, off, 64));
+}
+
 template <int K>
 __device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K */, double2* out /* K */) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if constexpr (K >= 2 && K <= 16 && (K & (K - 1)) == 0) {
+        // recursive halving: at the step with offset o the lane keeps the upper or
+        // lower half of its values (by lane bit o) and adds the partner's copy of
+        // that half, so K values need K-1 exchanges instead of K·6.  Afterwards lane
+        // l holds value k = (l >> s) & (K-1), s = 6 - log2 K, summed over the lanes
+        // that agree with l in bits s..5; the remaining bits are summed plainly.
+        int cnt = K;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double x = v[k].x, y = v[k].y;
+        for (int off = 32; off > 32 / K; off >>= 1) {
+            const int half = cnt / 2;
+            const bool up = (lane & off) != 0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            x += __shfl_xor(x, off, 64);
-            y += __shfl_xor(y, off, 64);
+            for (int i = 0; i < K / 2; ++i) {
+                if (i < half) {
+                    const double2 send = up ? v[i] : v[i + half];
+                    const double2 keep = up ? v[i + half] : v[i];
+                    v[i] = cadd(keep, shfl_xor2(send, off));
+                }
+            }
+            cnt = half;
         }
-        if (lane == 0) sRed[wv * K + k] = make_double2(x, y);
+        double2 s = v[0];
+#pragma unroll
+        for (int off = 32 / K; off > 0; off >>= 1) s = cadd(s, shfl_xor2(s, off));
+        constexpr int SH = (K == 2) ? 5 : (K == 4) ? 4 : (K == 8) ? 3 : 2;
+        if ((lane & ((1 << SH) - 1)) == 0) sRed[wv * K + ((lane >> SH) & (K - 1))] = s;
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double x = v[k].x, y = v[k].y;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                x += __shfl_xor(x, off, 64);
+                y += __shfl_xor(y, off, 64);
+            }
+            if (lane == 0) sRed[wv * K + k] = make_double2(x, y);
+        }
     }
     __syncthreads();
     if (threadIdx.x < K) {
