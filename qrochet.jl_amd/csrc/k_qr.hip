@@ -71,8 +71,41 @@ __device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K 
     __syncthreads();
 }
 
+// sum over the workgroup of one real value per thread, returned to every thread
+// (one barrier; `sRed4` must not be rewritten before the next barrier)
+__device__ __forceinline__ double block_sum_all(double x, double* sRed4) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) sRed4[wv] = x;
+    __syncthreads();
+    return (sRed4[0] + sRed4[1]) + (sRed4[2] + sRed4[3]);
+}
+
+// LAPACK zlarfg: H^H (alpha; x) = (beta; 0), H = I - tau v v^H, v = (1; x·scale), beta real
+struct Reflector {
+    double2 tau, scale;
+    double beta;
+};
+__device__ __forceinline__ Reflector make_reflector(double2 alpha, double xnorm2) {
+    Reflector h;
+    h.tau = cz();
+    h.scale = cz();
+    h.beta = alpha.x;
+    if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
+        const double nrm = sqrt(cabs2(alpha) + xnorm2);
+        h.beta = alpha.x >= 0.0 ? -nrm : nrm;
+        h.tau = make_double2((h.beta - alpha.x) / h.beta, -alpha.y / h.beta);
+        const double2 den = make_double2(alpha.x - h.beta, alpha.y);
+        const double dd = cabs2(den);
+        h.scale = make_double2(den.x / dd, -den.y / dd);
+    }
+    return h;
+}
+
 struct PanelShared {
     double2 sRed[4 * QR_NB];
+    double sRedN[2][4];
     double2 sDot[QR_NB];
     double2 sT[QR_NB][QR_NB + 1];
     double2 sScale;
@@ -192,37 +225,31 @@ __device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 
         sP[c][r] = (c < nb) ? it.a[(j0 + r) + (size_t)it.lda * (j0 + c)] : cz();
     }
     __syncthreads();
+    // Thread tid owns rows r = tid + 256 j for the whole panel, so the reflector's
+    // scaling, the dots, the trailing update and the next column's norm touch only
+    // the thread's own rows; the only cross-thread data are the reductions and
+    // alpha = sP[c][c], whose owner writes it before the barrier of the next norm.
+    double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
+    for (int r = tid; r < rows; r += 256)
+        if (r > 0) part += cabs2(sP[0][r]);
     for (int c = 0; c < nb; ++c) {
-        double2 part[1] = {cz()};
-        for (int r = c + 1 + tid; r < rows; r += 256) part[0].x += cabs2(sP[c][r]);
-        block_sum<1>(part, sh.sRed, sh.sDot);
-        if (tid == 0) {
-            const double xnorm2 = sh.sDot[0].x;
-            const double2 alpha = sP[c][c];
-            double2 tau = cz(), scale = cz();
-            double beta = alpha.x;
-            if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
-                const double nrm = sqrt(cabs2(alpha) + xnorm2);
-                beta = alpha.x >= 0.0 ? -nrm : nrm;
-                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-                const double2 den = make_double2(alpha.x - beta, alpha.y);
-                const double dd = cabs2(den);
-                scale = make_double2(den.x / dd, -den.y / dd);
-            }
-            sh.sTau[c] = tau;
-            sh.sScale = scale;
-            sP[c][c] = make_double2(beta, 0.0);
-        }
-        __syncthreads();
-        const double2 scale = sh.sScale;
-        for (int r = c + 1 + tid; r < rows; r += 256) sP[c][r] = cmul(sP[c][r], scale);  // v below the diagonal
-        __syncthreads();
-        // dots with v = (1 at r = c, sP[c][r] below)
+        // every thread forms the reflector from the reduced norm (no serial section);
+        // the two norm buffers alternate so a fast thread cannot overwrite partial
+        // sums that a slow thread of the previous column still reads
+        const double xnorm2 = block_sum_all(part, sh.sRedN[c & 1]);
+        const Reflector h = make_reflector(sP[c][c], xnorm2);
+        // v below the diagonal (scaled in place) and its dots with the panel:
+        // dots[k] = v^H a_k (k > c), dots[k] = v_k^H v (k < c)
         double2 dots[QR_NB];
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
-        for (int r = c + tid; r < rows; r += 256) {
-            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : sP[c][r];
+        for (int r = tid; r < rows; r += 256) {
+            if (r < c) continue;
+            double2 vr = make_double2(1.0, 0.0);
+            if (r > c) {
+                vr = cmul(sP[c][r], h.scale);
+                sP[c][r] = vr;
+            }
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
                 if (k > c && k < nb) dots[k] = cfmac(vr, sP[k][r], dots[k]);
@@ -233,27 +260,42 @@ __device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 
             }
         }
         block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
+        // (every thread has read alpha = sP[c][c] before the barriers above)
+        if (tid == c) sP[c][c] = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
             const int i = tid;
             double2 val = cz();
             if (i < c) {
                 double2 acc = cz();
                 for (int l = i; l < c; ++l) acc = cfma(sh.sT[i][l], sh.sDot[l], acc);
-                const double2 tc = sh.sTau[c];
+                const double2 tc = h.tau;
                 val = make_double2(-(tc.x * acc.x - tc.y * acc.y), -(tc.x * acc.y + tc.y * acc.x));
             } else if (i == c) {
-                val = sh.sTau[c];
+                val = h.tau;
             }
             sh.sT[i][c] = val;
         }
-        const double2 ctau = cconj(sh.sTau[c]);
-        for (int e = tid; e < (nb - c - 1) * (rows - c); e += 256) {
-            const int k = c + 1 + e / (rows - c), r = c + e % (rows - c);
+        // trailing update a_k -= conj(tau) (v^H a_k) v on own rows, and the norm of
+        // the next column below its diagonal
+        const double2 ctau = cconj(h.tau);
+        double2 w[QR_NB];
+#pragma unroll
+        for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? cmul(ctau, sh.sDot[k]) : cz();
+        part = 0.0;
+        for (int r = tid; r < rows; r += 256) {
+            if (r < c) continue;
             const double2 vr = (r == c) ? make_double2(1.0, 0.0) : sP[c][r];
-            sP[k][r] = csub(sP[k][r], cmul(cmul(ctau, sh.sDot[k]), vr));
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) {
+                if (k > c && k < nb) {
+                    const double2 x = csub(sP[k][r], cmul(w[k], vr));
+                    sP[k][r] = x;
+                    if (k == c + 1 && r > c + 1) part += cabs2(x);
+                }
+            }
         }
-        __syncthreads();
     }
+    __syncthreads();
     // write R (upper part) back to a, the explicit reflectors to v
     for (int e = tid; e < nb * rows; e += 256) {
         const int c = e / rows, r = e % rows;

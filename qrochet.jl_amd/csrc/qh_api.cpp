@@ -83,7 +83,8 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
     if (count <= 0) return;
     if (!ms || !bonds || !gates) throw Error(QH_ERR_ARGUMENT, "null argument");
     const unsigned prec = flags & QH_PREC_MASK;
-    if (prec != QH_PREC_FP64) throw Error(QH_ERR_ARGUMENT, "requested contraction precision is not available");
+    if (prec != QH_PREC_FP64 && prec != QH_PREC_FP32 && prec != QH_PREC_BF16X3)
+        throw Error(QH_ERR_ARGUMENT, "unknown contraction precision");
     const bool canonical = flags & QH_CANONICAL;
     const bool renorm = flags & QH_RENORMALIZE;
     auto thr_of = [&](int i) { return thresholds ? thresholds[i] : std::nan(""); };
@@ -150,7 +151,7 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
         s.outs = {u, v};
         s.lam_dst = nlam[i];
     }
-    run_theta(c, th);
+    run_theta(c, th, (int)prec);
     run_svd_tasks(c, tasks);
     for (int i = 0; i < count; ++i) {
         qh_mps* m = ms[i];

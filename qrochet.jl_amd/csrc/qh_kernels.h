@@ -35,6 +35,11 @@ struct ThetaItem {
 // values of l and of r.
 void launch_theta(const ThetaItem* d_items, const int2* d_tiles, int ntiles, int d, hipStream_t s);
 int theta_tile_extent(int d);
+// Reduced-precision build on MFMA (precision = QH_PREC_FP32 or QH_PREC_BF16X3), FP64
+// gate mix and output; tiles cover 32 values of l and of r.
+void launch_theta_mfma(const ThetaItem* d_items, const int2* d_tiles, int ntiles, int d, int precision,
+                       hipStream_t s);
+constexpr int THETA_MFMA_TILE = 32;
 
 // ---------------------------------------------------------------------------
 // batched complex GEMM:  C = alpha · op(A) · diag(dk) · op(B) + beta · C

@@ -130,10 +130,11 @@ Timed::~Timed() {
 // ---------------------------------------------------------------------------
 // θ build
 // ---------------------------------------------------------------------------
-void run_theta(qh_context* c, std::vector<ThetaItem>& items) {
+void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision) {
     if (items.empty()) return;
     const int d = items[0].d;
-    const int te = theta_tile_extent(d);
+    const bool mfma = precision != QH_PREC_FP64;
+    const int te = mfma ? THETA_MFMA_TILE : theta_tile_extent(d);
     std::vector<int2> tiles;
     double flops = 0, bytes = 0;
     for (size_t i = 0; i < items.size(); ++i) {
@@ -148,7 +149,8 @@ void run_theta(qh_context* c, std::vector<ThetaItem>& items) {
     ThetaItem* di = upload(c, items);
     int2* dt = upload(c, tiles);
     Timed tm(c, "theta", flops, bytes);
-    launch_theta(di, dt, (int)tiles.size(), d, c->stream);
+    if (mfma) launch_theta_mfma(di, dt, (int)tiles.size(), d, precision, c->stream);
+    else launch_theta(di, dt, (int)tiles.size(), d, c->stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -104,7 +104,7 @@ struct Timed {
 };
 
 // ---- batched building blocks ----
-void run_theta(qh_context* c, std::vector<ThetaItem>& items);
+void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_PREC_FP64);
 void run_gemm(qh_context* c, std::vector<GemmItem>& items);
 // runs set-identity + sweeps until every item converged; throws on sweep limit
 void run_jacobi(qh_context* c, std::vector<JacItem>& items);
