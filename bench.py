@@ -97,6 +97,43 @@ def serving_leg(kind, requests, qps, seed=2025):
     return out
 
 
+def batched_leg(qh, wl, precision, count, depth):
+    """Batched uneven workload: `count` independent MPS (n ~ U{8..64}, χmax uniform
+    over {16..256}, random left-canonized initial states at the χmax profile),
+    Haar brickwork of `depth` layers; each layer of ALL MPS is one batched call."""
+    items = wl.config3(count=count, depth=depth)
+    log(f"batched workload: {count} MPS generated")
+    ctx = qh.default_context()
+    states = [qh.from_internal(it["arrays"]) for it in items]
+    ctx.synchronize()
+    n_up = sum(len(l) for it in items for l in it["layers"])
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for layer in range(depth):
+        ups, mds = [], []
+        for st, it in zip(states, items):
+            for b, g in it["layers"][layer]:
+                ups.append((st, qh.gate(g, b, b + 1)))
+                mds.append(it["chi"])
+        qh.evolve_layer_(ups, maxdim=mds, precision=precision)
+        ctx.synchronize()
+        log(f"layer {layer}: {len(ups)} updates, {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    classes = ["jacobi_round", "theta", "qr", "qr_apply", "select", "gather"]
+    return {
+        "metric": "2-site gate+SVD updates/s, batched uneven workload (one launch pipeline per layer of all MPS)",
+        "value": round(n_up / elapsed, 2), "unit": "updates/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"{count} open MPS, n~U{{8..64}}, d=2, chimax uniform in {{16,32,64,128,256}}, "
+                               f"random left-canonized states at the chimax profile, Haar brickwork depth {depth}, "
+                               f"maxdim=chimax, threshold default", "precision": precision,
+                   "updates": n_up, "wall_s": round(elapsed, 3)},
+        "kernel_ms": {k: round(ctx.timing(k)["ms"], 2) for k in classes},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +148,12 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run only the request loop, print its own line")
     ap.add_argument("--serving-requests", type=int, default=None)
     ap.add_argument("--qps", type=float, default=None)
+    ap.add_argument("--workload", choices=("tebd", "batched"), default="tebd",
+                    help="tebd: the n=64, chi=256 Trotter evolution (default); batched: 512 uneven MPS")
+    ap.add_argument("--precision", choices=("fp64", "fp32", "bf16x3"), default=None,
+                    help="contraction precision of the batched workload (default fp32)")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-depth", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 1 GiB device copy (known bytes) for HBM counter calibration")
     args = ap.parse_args()
@@ -132,6 +175,12 @@ def main():
         log("calibration copy: 3 launches, each 1 GiB read + 1 GiB written (torch elementwise mul by 1)")
         del src, dst
 
+    if args.workload == "batched":
+        from qrochet_hip import workloads as wl
+
+        print(json.dumps(batched_leg(qh, wl, args.precision or "fp32", args.batch_count, args.batch_depth)))
+        return
+
     if args.serving != "off":
         full = args.serving == "full"
         n_req = args.serving_requests or (40 if full else 48)
@@ -149,7 +198,7 @@ def main():
         }))
         return
 
-    from oracle import workloads as wl
+    from qrochet_hip import workloads as wl
 
     arrays, layers, trunc, _ = wl.config2(steps=args.trotter)
     n_updates = sum(len(l) for l in layers)
@@ -216,6 +265,24 @@ def main():
     ctx.synchronize()
     tb = time.perf_counter() - tb0
     extra["saturated_batched_updates_per_s"] = round(sample_updates / tb, 2)
+    z_terms = [[qh.gate(wl.Z, i)] for i in range(1, psi_b.n + 1)]
+    z64 = qh.expect_many(psi_b, z_terms)
+    mixed = {}
+    for prec in ("fp32", "bf16x3"):
+        psi_m = state_sat.copy()
+        ctx.synchronize()
+        tm0 = time.perf_counter()
+        for layer in sample:
+            qh.evolve_layer_([(psi_m, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
+                             maxdim=trunc["maxdim"], precision=prec)
+        ctx.synchronize()
+        tm = time.perf_counter() - tm0
+        zm = qh.expect_many(psi_m, z_terms)
+        mixed[prec] = {"updates_per_s": round(sample_updates / tm, 2),
+                       "max_abs_dZ_vs_fp64": float(np.max(np.abs(zm - z64))),
+                       "sum_bond_dims": int(sum(psi_m.bond_dims())),
+                       "sum_bond_dims_fp64": int(sum(psi_b.bond_dims()))}
+    extra["mixed_precision_sample"] = mixed
     if not args.no_plain:
         psi_p = state_sat.copy()
         ctx.synchronize()

@@ -28,6 +28,7 @@ int guard(qh_context* c, F&& f) {
         if (c) {
             QH_HIP(hipSetDevice(c->device));
             c->call_allocs.clear();
+            arena_recycle(c);
         }
         f();
         if (c) {
@@ -578,14 +579,16 @@ int qh_ctx_create(int device, qh_context** out) {
         c = new qh_context();
         c->device = device;
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        // a small first chunk makes tests exercise the multi-chunk arena cheaply
+        if (const char* e = std::getenv("QH_ARENA_INITIAL_BYTES")) {
+            const long long v = std::atoll(e);
+            if (v >= 256) c->arena_first = (size_t)v;
+        }
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t thr = UINT64_MAX;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
         }
-        c->arena_cap = size_t(4) << 20;
-        QH_HIP(hipHostMalloc((void**)&c->h_arena, c->arena_cap, hipHostMallocDefault));
-        QH_HIP(hipMalloc((void**)&c->d_arena, c->arena_cap));
     });
     if (rc != QH_OK) {
         if (c) {
@@ -604,8 +607,7 @@ int qh_ctx_destroy(qh_context* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& p : c->pending) { c->event_pool.push_back(p.a); c->event_pool.push_back(p.b); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
-    if (c->h_arena) (void)hipHostFree(c->h_arena);
-    if (c->d_arena) (void)hipFree(c->d_arena);
+    arena_free(c);
     if (c->h_read) (void)hipHostFree(c->h_read);
     (void)hipStreamDestroy(c->stream);
     delete c;

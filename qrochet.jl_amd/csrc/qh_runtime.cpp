@@ -32,7 +32,6 @@ static void collect_timers(qh_context* c) {
 
 void ctx_sync(qh_context* c) {
     QH_HIP(hipStreamSynchronize(c->stream));
-    c->arena_off = 0;
     if (!c->pending.empty()) collect_timers(c);
 }
 
@@ -63,27 +62,49 @@ void release_call_allocs(qh_context* c) {
 
 void* arena_upload(qh_context* c, const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
-    if (c->arena_off + need > c->arena_cap) {
-        ctx_sync(c);  // every earlier upload has been consumed
-        if (need > c->arena_cap) {
-            size_t cap = std::max(need, c->arena_cap * 2);
-            if (c->h_arena) (void)hipHostFree(c->h_arena);
-            if (c->d_arena) (void)hipFree(c->d_arena);
-            c->h_arena = nullptr;
-            c->d_arena = nullptr;
-            QH_HIP(hipHostMalloc((void**)&c->h_arena, cap, hipHostMallocDefault));
-            QH_HIP(hipMalloc((void**)&c->d_arena, cap));
-            c->arena_cap = cap;
+    if (c->arena.empty() || c->arena_off + need > c->arena.back().cap) {
+        qh_context::ArenaChunk ch;
+        ch.cap = std::max(need, c->arena.empty() ? c->arena_first : c->arena.back().cap * 2);
+        QH_HIP(hipHostMalloc((void**)&ch.h, ch.cap, hipHostMallocDefault));
+        if (hipMalloc((void**)&ch.d, ch.cap) != hipSuccess) {
+            (void)hipHostFree(ch.h);
+            throw Error(QH_ERR_OOM, "descriptor arena allocation failed");
         }
+        c->arena.push_back(ch);
+        c->arena_off = 0;
     }
-    char* h = c->h_arena + c->arena_off;
-    char* d = c->d_arena + c->arena_off;
+    char* h = c->arena.back().h + c->arena_off;
+    char* d = c->arena.back().d + c->arena_off;
     if (bytes) {
         std::memcpy(h, src, bytes);
         QH_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
     }
     c->arena_off += need;
     return d;
+}
+
+void arena_recycle(qh_context* c) {
+    if (c->arena.empty() || (c->arena.size() == 1 && c->arena_off == 0)) return;
+    // every upload so far has been consumed once the stream is idle
+    QH_HIP(hipStreamSynchronize(c->stream));
+    if (c->arena.size() > 1) {
+        // the last chunk is the largest (capacities double); keep it
+        for (size_t i = 0; i + 1 < c->arena.size(); ++i) {
+            (void)hipHostFree(c->arena[i].h);
+            (void)hipFree(c->arena[i].d);
+        }
+        c->arena.erase(c->arena.begin(), c->arena.end() - 1);
+    }
+    c->arena_off = 0;
+}
+
+void arena_free(qh_context* c) {
+    for (auto& ch : c->arena) {
+        (void)hipHostFree(ch.h);
+        (void)hipFree(ch.d);
+    }
+    c->arena.clear();
+    c->arena_off = 0;
 }
 
 void readback(qh_context* c, void* host, const void* dev, size_t bytes) {

@@ -36,10 +36,18 @@ struct qh_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    // descriptor arena: pinned host staging + device mirror, reset at every sync
-    char* h_arena = nullptr;
-    char* d_arena = nullptr;
-    size_t arena_cap = 0, arena_off = 0;
+    // descriptor arena: chunks of pinned host staging + device mirror.  Within one
+    // API call uploads only append (a full chunk opens a new one), so a descriptor
+    // stays valid for the rest of the call; chunks are recycled at the entry of
+    // the next call, after the stream has drained.
+    struct ArenaChunk {
+        char* h = nullptr;
+        char* d = nullptr;
+        size_t cap = 0;
+    };
+    std::vector<ArenaChunk> arena;
+    size_t arena_off = 0;            // offset in arena.back()
+    size_t arena_first = 4u << 20;   // capacity of the first chunk
     // small pinned readback area
     char* h_read = nullptr;
     size_t read_cap = 0;
@@ -85,8 +93,12 @@ void dfree(qh_context* c, void* p);
 void keep(qh_context* c, void* p);
 // free what the failed call left behind
 void release_call_allocs(qh_context* c);
-// copy host bytes to the descriptor arena; returns a device pointer valid until the next sync
+// copy host bytes to the descriptor arena; the device pointer stays valid until the
+// calling API entry point returns
 void* arena_upload(qh_context* c, const void* src, size_t bytes);
+// at API entry: drain the stream and recycle the arena (keeps the largest chunk)
+void arena_recycle(qh_context* c);
+void arena_free(qh_context* c);
 template <class T>
 T* upload(qh_context* c, const std::vector<T>& v) {
     return static_cast<T*>(arena_upload(c, v.data(), v.size() * sizeof(T)));

@@ -348,3 +348,31 @@ def test_rank_deficient_theta_without_truncation(qh):
     oc.evolve(ref, oc.gate2(g, 2))
     assert psi.bond_dims() == ref.bond_dims()
     assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-12
+
+
+def test_batched_layer_with_multichunk_descriptor_arena(qh, monkeypatch):
+    """Descriptors of one batched call that overflow the arena's first chunk stay
+    valid for the whole call (regression: a full arena once reallocated or wrapped
+    in the middle of a call and invalidated descriptors uploaded before)."""
+    monkeypatch.setenv("QH_ARENA_INITIAL_BYTES", "4096")
+    ctx = qh.Context()
+    rng = np.random.default_rng(17)
+    states, ups, ref_states, ref_ups = [], [], [], []
+    for k in range(12):
+        n = 10
+        arrays = _random_chain(rng, n, 2, 16)
+        a = qh.from_internal(arrays, context=ctx)
+        b = qh.from_internal(arrays, context=ctx)
+        for bd in wl.brickwork_bonds(n, k % 2):
+            g = wl.haar_gate4(rng)
+            ups.append((a, qh.gate(g, bd, bd + 1)))
+            ref_ups.append((b, qh.gate(g, bd, bd + 1)))
+        states.append(a)
+        ref_states.append(b)
+    qh.evolve_layer_(ups, maxdim=16, context=ctx)   # one call, descriptors span several chunks
+    for st, g in ref_ups:
+        qh.evolve_(st, g, maxdim=16)
+    for a, b in zip(states, ref_states):
+        assert a.bond_dims() == b.bond_dims()
+        for x, y in zip(a.internal_arrays(), b.internal_arrays()):
+            assert np.array_equal(x, y)
