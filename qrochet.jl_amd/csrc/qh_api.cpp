@@ -608,6 +608,7 @@ int qh_ctx_destroy(qh_context* c) {
     for (auto& p : c->pending) { c->event_pool.push_back(p.a); c->event_pool.push_back(p.b); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     arena_free(c);
+    pool_destroy(c);
     if (c->h_read) (void)hipHostFree(c->h_read);
     (void)hipStreamDestroy(c->stream);
     delete c;

@@ -35,12 +35,67 @@ void ctx_sync(qh_context* c) {
     if (!c->pending.empty()) collect_timers(c);
 }
 
+namespace {
+
+size_t pool_class(size_t bytes) {
+    if (bytes <= 4096) return (bytes + 255) & ~size_t(255);
+    size_t base = 1;
+    while ((base << 1) <= bytes) base <<= 1;
+    const size_t step = base / 4;
+    return (bytes + step - 1) / step * step;
+}
+
+constexpr size_t POOL_SLAB = size_t(8) << 20;
+constexpr size_t POOL_SMALL = size_t(1) << 20;
+
+void* pool_alloc(qh_context* c, size_t bytes) {
+    auto& P = c->pool;
+    const size_t cs = pool_class(bytes);
+    auto& fl = P.free[cs];
+    if (fl.empty()) {
+        auto grab = [&](size_t n) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, n) != hipSuccess || !p) {
+                (void)hipGetLastError();
+                pool_trim(c);
+                p = nullptr;
+                if (hipMalloc(&p, n) != hipSuccess || !p) {
+                    (void)hipGetLastError();
+                    return nullptr;
+                }
+            }
+            P.reserved += n;
+            return p;
+        };
+        if (cs <= POOL_SMALL) {
+            char* slab = static_cast<char*>(grab(POOL_SLAB));
+            if (!slab) return nullptr;
+            P.slabs.push_back(slab);
+            auto& fl2 = P.free[cs];  // (pool_trim may have rehashed the map)
+            for (size_t off = 0; off + cs <= POOL_SLAB; off += cs) {
+                fl2.push_back(slab + off);
+                P.size_of[slab + off] = cs;
+            }
+        } else {
+            void* b = grab(cs);
+            if (!b) return nullptr;
+            P.big.push_back(b);
+            P.size_of[b] = cs;
+            P.free[cs].push_back(b);
+        }
+    }
+    auto& fl3 = P.free[cs];
+    void* p = fl3.back();
+    fl3.pop_back();
+    return p;
+}
+
+}  // namespace
+
 void* dalloc_bytes(qh_context* c, size_t bytes) {
-    void* p = nullptr;
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMallocAsync(&p, bytes, c->stream);
-    if (e != hipSuccess || p == nullptr)
-        throw Error(QH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    void* p = pool_alloc(c, bytes);
+    if (!p) throw Error(QH_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     c->call_allocs.insert(p);
     return p;
 }
@@ -48,7 +103,40 @@ void* dalloc_bytes(qh_context* c, size_t bytes) {
 void dfree(qh_context* c, void* p) {
     if (!p) return;
     c->call_allocs.erase(p);
-    (void)hipFreeAsync(p, c->stream);
+    auto it = c->pool.size_of.find(p);
+    if (it != c->pool.size_of.end()) c->pool.free[it->second].push_back(p);
+}
+
+void pool_trim(qh_context* c) {
+    // only whole big blocks can go back; slabs stay (their blocks may be in use)
+    auto& P = c->pool;
+    (void)hipStreamSynchronize(c->stream);
+    std::unordered_set<void*> freed;
+    for (auto& kv : P.free) {
+        if (kv.first <= POOL_SMALL) continue;
+        for (void* b : kv.second) {
+            (void)hipFree(b);
+            P.size_of.erase(b);
+            P.reserved -= kv.first;
+            freed.insert(b);
+        }
+        kv.second.clear();
+    }
+    std::vector<void*> keep_big;
+    for (void* b : P.big)
+        if (!freed.count(b)) keep_big.push_back(b);
+    P.big.swap(keep_big);
+}
+
+void pool_destroy(qh_context* c) {
+    auto& P = c->pool;
+    for (void* b : P.big) (void)hipFree(b);
+    for (void* s : P.slabs) (void)hipFree(s);
+    P.big.clear();
+    P.slabs.clear();
+    P.free.clear();
+    P.size_of.clear();
+    P.reserved = 0;
 }
 
 void keep(qh_context* c, void* p) {
@@ -56,7 +144,8 @@ void keep(qh_context* c, void* p) {
 }
 
 void release_call_allocs(qh_context* c) {
-    for (void* p : c->call_allocs) (void)hipFreeAsync(p, c->stream);
+    std::vector<void*> v(c->call_allocs.begin(), c->call_allocs.end());
+    for (void* p : v) dfree(c, p);
     c->call_allocs.clear();
 }
 
