@@ -34,6 +34,7 @@ extern "C" {
 
 typedef struct qh_context qh_context;
 typedef struct qh_mps qh_mps;
+typedef struct qh_mpo qh_mpo;
 
 typedef enum {
     QH_OK = 0,
@@ -166,6 +167,26 @@ int qh_expect_many(qh_context* ctx, qh_mps* mps, int nterms, const int* offsets,
 int qh_overlap(qh_context* ctx, qh_mps* a, qh_mps* b, double* out);
 /* `norm(ψ)` (src/Ansatz.jl:101-109). */
 int qh_norm(qh_context* ctx, qh_mps* mps, double* out);
+
+/* ---- MPO × MPS ------------------------------------------------------------ */
+/* Replaces `MPO(arrays)` = `Chain(Operator(), Open(), arrays)` (src/Ansatz/Chain.jl:20,133-172)
+ * followed by `adapt` to the device.  arrays[i]: (o, i, l, r) column-major (the
+ * reference's default order, Chain.jl:34), boundary tensors 3-D: (o, i, r) for the
+ * first site, (o, i, l) for the last.  bond_dims: n-1 entries. */
+int qh_mpo_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out);
+int qh_mpo_free(qh_mpo* mpo);
+/* H|ψ⟩ exactly: a new MPS with bond dimensions χ_b·D_b (Λ vectors of ψ absorbed).
+ * The reference has no apply routine: it would `merge` the MPO and MPS networks
+ * (src/Quantum.jl:330-348) and contract them; this is the same state as a Chain. */
+int qh_mpo_apply(qh_context* ctx, const qh_mpo* mpo, const qh_mps* psi, qh_mps** out);
+/* Variational compression (no reference counterpart): φ <- argmin ‖φ - H ψ‖ over MPS
+ * with φ's bond dimensions, by `sweeps` left-right-left one-site fitting sweeps.  φ
+ * (distinct from ψ) is the initial guess; its Λ vectors are absorbed and on return
+ * it is right-orthonormal from site 2 with the orthogonality centre on site 1.
+ * flags: QH_PREC_* for the environment contractions (FP64 vector ALU, FP32 or bf16x3
+ * on MFMA); orthonormalisation stays FP64.  overlap: 2 doubles, ⟨φ|Hψ⟩ at the end. */
+int qh_mpo_fit(qh_context* ctx, const qh_mpo* mpo, const qh_mps* psi, qh_mps* phi, int sweeps, unsigned flags,
+               double* overlap);
 
 #ifdef __cplusplus
 }

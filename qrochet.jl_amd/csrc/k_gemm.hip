@@ -154,12 +154,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ i
         for (int s = 0; s < 4; ++s) {
             const int e = tid + 256 * s;
             int row, kk;
-            if (g.opa == 0) { row = e & 63; kk = e >> 6; }
+            if (op_plain(g.opa)) { row = e & 63; kk = e >> 6; }
             else { kk = e & 15; row = e >> 4; }
             const int gr = m0 + row, gk = k0 + kk;
             double2 v = cz();
             if (gr < g.m && gk < g.k) {
-                v = g.opa == 0 ? g.a[gr + (size_t)g.lda * gk] : cconj(g.a[gk + (size_t)g.lda * gr]);
+                v = load_op(g.a, g.lda, gr, gk, g.opa);
                 if (g.dk) v = cscale(v, g.dk[(gk / g.dk_div) % g.dk_mod]);
             }
             sA[kk][row] = v;
@@ -168,12 +168,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ i
         for (int s = 0; s < 4; ++s) {
             const int e = tid + 256 * s;
             int col, kk;
-            if (g.opb == 0) { kk = e & 15; col = e >> 4; }
+            if (op_plain(g.opb)) { kk = e & 15; col = e >> 4; }
             else { col = e & 63; kk = e >> 6; }
             const int gc = n0 + col, gk = k0 + kk;
             double2 v = cz();
-            if (gc < g.n && gk < g.k)
-                v = g.opb == 0 ? g.b[gk + (size_t)g.ldb * gc] : cconj(g.b[gc + (size_t)g.ldb * gk]);
+            if (gc < g.n && gk < g.k) v = load_op(g.b, g.ldb, gk, gc, g.opb);
             sB[kk][col] = v;
         }
         __syncthreads();

@@ -198,4 +198,146 @@ void launch_theta_mfma(const ThetaItem* d_items, const int2* d_tiles, int ntiles
     }
 }
 
+// ---------------------------------------------------------------------------
+// Generic batched complex GEMM on MFMA (same items and tiles as k_gemm_f64):
+// C = alpha · op(A) diag(dk) op(B) (+ beta C), rows × rowscale, columns × colscale.
+// A 64×64 tile per workgroup; wave w computes the 32×32 quarter (w % 2, w / 2)
+// as 2×2 MFMA blocks of 16×16.  Operands are staged through LDS in FP32.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ items,
+                                                   const int2* __restrict__ tiles) {
+    constexpr int TM = GEMM_TM, TN = GEMM_TN, TK = GEMM_TK;
+    __shared__ float2 sA[TK][TM];
+    __shared__ float2 sB[TK][TN];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int2 tt = tiles[blockIdx.x];
+    const GemmItem g = items[tt.x];
+    const int m0 = (tt.y % g.tiles_m) * TM, n0 = (tt.y / g.tiles_m) * TN;
+    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+    const int li = lane & 15, kq = lane >> 4;
+
+    f32x4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accR[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            accI[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+
+    for (int k0 = 0; k0 < g.k; k0 += TK) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int e = tid + 256 * s;
+            int row, kk;
+            if (op_plain(g.opa)) { row = e & 63; kk = e >> 6; }
+            else { kk = e & 15; row = e >> 4; }
+            const int gr = m0 + row, gk = k0 + kk;
+            double2 v = cz();
+            if (gr < g.m && gk < g.k) {
+                v = load_op(g.a, g.lda, gr, gk, g.opa);
+                if (g.dk) v = cscale(v, g.dk[(gk / g.dk_div) % g.dk_mod]);
+            }
+            sA[kk][row] = make_float2((float)v.x, (float)v.y);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int e = tid + 256 * s;
+            int col, kk;
+            if (op_plain(g.opb)) { kk = e & 15; col = e >> 4; }
+            else { col = e & 63; kk = e >> 6; }
+            const int gc = n0 + col, gk = k0 + kk;
+            double2 v = cz();
+            if (gc < g.n && gk < g.k) v = load_op(g.b, g.ldb, gk, gc, g.opb);
+            sB[kk][col] = make_float2((float)v.x, (float)v.y);
+        }
+        __syncthreads();
+        if constexpr (P == 1) {
+#pragma unroll
+            for (int st = 0; st < TK / 4; ++st) {
+                const int kk = kq + 4 * st;
+                float2 av[2], bv[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    av[t] = sA[kk][wm + 16 * t + li];
+                    bv[t] = sB[kk][wn + 16 * t + li];
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a].x, bv[b].x, accR[a][b], 0, 0, 0);
+                        accR[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[a].y, bv[b].y, accR[a][b], 0, 0, 0);
+                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a].x, bv[b].y, accI[a][b], 0, 0, 0);
+                        accI[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a].y, bv[b].x, accI[a][b], 0, 0, 0);
+                    }
+            }
+        } else {
+            bf16x4 arh[2], arl[2], aih[2], ail[2], ainh[2], ainl[2];
+            bf16x4 brh[2], brl[2], bih[2], bil[2];
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float2 a = sA[4 * kq + t][wm + 16 * t2 + li];
+                    const float2 b = sB[4 * kq + t][wn + 16 * t2 + li];
+                    short h, l;
+                    bf16_split(a.x, h, l); arh[t2][t] = h; arl[t2][t] = l;
+                    bf16_split(a.y, h, l); aih[t2][t] = h; ail[t2][t] = l;
+                    ainh[t2][t] = (short)(h ^ (short)0x8000); ainl[t2][t] = (short)(l ^ (short)0x8000);
+                    bf16_split(b.x, h, l); brh[t2][t] = h; brl[t2][t] = l;
+                    bf16_split(b.y, h, l); bih[t2][t] = h; bil[t2][t] = l;
+                }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    f32x4 r = accR[a][b], i = accI[a][b];
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[a], brh[b], r, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[a], brl[b], r, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl[a], brh[b], r, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ainh[a], bih[b], r, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ainh[a], bil[b], r, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ainl[a], bih[b], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[a], bih[b], i, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh[a], bil[b], i, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl[a], bih[b], i, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih[a], brh[b], i, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih[a], brl[b], i, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ail[a], brh[b], i, 0, 0, 0);
+                    accR[a][b] = r;
+                    accI[a][b] = i;
+                }
+        }
+        __syncthreads();
+    }
+    const bool use_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int c = n0 + wn + 16 * b + li;
+            if (c >= g.n) continue;
+            const double csc = g.colscale ? g.colscale[c] : 1.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = m0 + wm + 16 * a + 4 * kq + t;
+                if (r >= g.m) continue;
+                double2 v = cmul(g.alpha, make_double2((double)accR[a][b][t], (double)accI[a][b][t]));
+                v = cscale(v, csc * (g.rowscale ? g.rowscale[r] : 1.0));
+                double2* dst = g.c + r + (size_t)g.ldc * c;
+                if (use_beta) v = cfma(g.beta, *dst, v);
+                *dst = v;
+            }
+        }
+}
+
+void launch_gemm_mfma(const GemmItem* d_items, const int2* d_tiles, int ntiles, int precision, hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (precision == QH_PREC_FP32) k_gemm_mfma<1><<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+    else k_gemm_mfma<2><<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
 }  // namespace qh

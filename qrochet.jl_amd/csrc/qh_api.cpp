@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "qh_runtime.h"
@@ -20,35 +21,9 @@ void launch_apply_op(cplx* t, int left, int dim, int right, const cplx* op, hipS
 
 namespace {
 
-thread_local std::string g_err;
-
 template <class F>
 int guard(qh_context* c, F&& f) {
-    try {
-        if (c) {
-            QH_HIP(hipSetDevice(c->device));
-            c->call_allocs.clear();
-            arena_recycle(c);
-        }
-        f();
-        if (c) {
-            c->err.clear();
-            c->call_allocs.clear();
-        }
-        return QH_OK;
-    } catch (const Error& e) {
-        if (c) release_call_allocs(c);
-        (c ? c->err : g_err) = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (c) release_call_allocs(c);
-        (c ? c->err : g_err) = "host allocation failed";
-        return QH_ERR_OOM;
-    } catch (const std::exception& e) {
-        if (c) release_call_allocs(c);
-        (c ? c->err : g_err) = e.what();
-        return QH_ERR_HIP;
-    }
+    return api_guard(c, std::forward<F>(f));
 }
 
 using zc = std::complex<double>;
@@ -569,7 +544,7 @@ std::vector<cplx*> left_envs(qh_context* c, const qh_mps* ket, const qh_mps* bra
 extern "C" {
 
 int qh_ctx_create(int device, qh_context** out) {
-    if (!out) { g_err = "null output pointer"; return QH_ERR_ARGUMENT; }
+    if (!out) { thread_error() = "null output pointer"; return QH_ERR_ARGUMENT; }
     qh_context* c = nullptr;
     int rc = guard(nullptr, [&] {
         int ndev = 0;
@@ -620,7 +595,7 @@ int qh_ctx_synchronize(qh_context* c) {
     return guard(c, [&] { ctx_sync(c); });
 }
 
-const char* qh_last_error(const qh_context* c) { return c ? c->err.c_str() : g_err.c_str(); }
+const char* qh_last_error(const qh_context* c) { return c ? c->err.c_str() : thread_error().c_str(); }
 
 int qh_ctx_set_timing(qh_context* c, int enable) {
     if (!c) return QH_ERR_ARGUMENT;

@@ -54,11 +54,26 @@ struct GemmItem {
     const double* colscale;  // N
     cplx alpha, beta;
     int m, n, k, lda, ldb, ldc;
-    int opa, opb;            // 0 = N, 1 = conjugate transpose
+    int opa, opb;            // GEMM_N, GEMM_C (conjugate transpose), GEMM_T (transpose), GEMM_CONJ
     int dk_mod, dk_div;
     int tiles_m, tiles_n;
 };
+enum { GEMM_N = 0, GEMM_C = 1, GEMM_T = 2, GEMM_CONJ = 3 };
+// element (r, c) of op(X) where X is stored column-major with leading dimension ld
+__device__ __forceinline__ bool op_plain(int op) { return op == GEMM_N || op == GEMM_CONJ; }
+__device__ __forceinline__ cplx load_op(const cplx* x, int ld, int r, int c, int op) {
+    if (op_plain(op)) {
+        const cplx v = x[r + (size_t)ld * c];
+        return op == GEMM_CONJ ? make_double2(v.x, -v.y) : v;
+    }
+    const cplx v = x[c + (size_t)ld * r];
+    return op == GEMM_C ? make_double2(v.x, -v.y) : v;
+}
+// FP64 on the vector ALUs
 void launch_gemm(const GemmItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// FP32 or bf16x3 products on MFMA with FP32 accumulation (precision = QH_PREC_FP32 /
+// QH_PREC_BF16X3), FP64 epilogue; same tiles as launch_gemm
+void launch_gemm_mfma(const GemmItem* d_items, const int2* d_tiles, int ntiles, int precision, hipStream_t s);
 constexpr int GEMM_TM = 64, GEMM_TN = 64, GEMM_TK = 16;
 
 // ---------------------------------------------------------------------------

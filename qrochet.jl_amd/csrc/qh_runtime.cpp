@@ -30,6 +30,11 @@ static void collect_timers(qh_context* c) {
     c->pending.clear();
 }
 
+std::string& thread_error() {
+    static thread_local std::string e;
+    return e;
+}
+
 void ctx_sync(qh_context* c) {
     QH_HIP(hipStreamSynchronize(c->stream));
     if (!c->pending.empty()) collect_timers(c);
@@ -280,7 +285,7 @@ GemmItem gemm_item(const cplx* a, int opa, int lda, const cplx* b, int opb, int 
     return g;
 }
 
-void run_gemm(qh_context* c, std::vector<GemmItem>& items) {
+void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
     if (items.empty()) return;
     std::vector<int2> tiles;
     double flops = 0, bytes = 0;
@@ -298,7 +303,8 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items) {
     GemmItem* di = upload(c, items);
     int2* dt = upload(c, tiles);
     Timed tm(c, "gemm", flops, bytes);
-    launch_gemm(di, dt, (int)tiles.size(), c->stream);
+    if (precision == QH_PREC_FP64) launch_gemm(di, dt, (int)tiles.size(), c->stream);
+    else launch_gemm_mfma(di, dt, (int)tiles.size(), precision, c->stream);
 }
 
 // ---------------------------------------------------------------------------

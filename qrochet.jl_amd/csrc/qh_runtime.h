@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <new>
 #include <unordered_map>
 #include <unordered_set>
 #include <stdexcept>
@@ -135,7 +136,7 @@ struct Timed {
 
 // ---- batched building blocks ----
 void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_PREC_FP64);
-void run_gemm(qh_context* c, std::vector<GemmItem>& items);
+void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);
 // runs set-identity + sweeps until every item converged; throws on sweep limit
 void run_jacobi(qh_context* c, std::vector<JacItem>& items);
 void run_select(qh_context* c, std::vector<SelectItem>& items);
@@ -192,5 +193,41 @@ struct QrApplyJob {
     int ncols;
 };
 void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs);
+
+
+// ---- entry-point wrapper ----
+// error text of calls made without a context (per thread)
+std::string& thread_error();
+// Runs one C-ABI entry point: device selection, per-call allocation tracking and
+// arena recycling; exceptions become status codes, and a failed call releases
+// the device buffers it allocated.
+template <class F>
+int api_guard(qh_context* c, F&& f) {
+    try {
+        if (c) {
+            QH_HIP(hipSetDevice(c->device));
+            c->call_allocs.clear();
+            arena_recycle(c);
+        }
+        f();
+        if (c) {
+            c->err.clear();
+            c->call_allocs.clear();
+        }
+        return QH_OK;
+    } catch (const Error& e) {
+        if (c) release_call_allocs(c);
+        (c ? c->err : thread_error()) = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) release_call_allocs(c);
+        (c ? c->err : thread_error()) = "host allocation failed";
+        return QH_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (c) release_call_allocs(c);
+        (c ? c->err : thread_error()) = e.what();
+        return QH_ERR_HIP;
+    }
+}
 
 }  // namespace qh

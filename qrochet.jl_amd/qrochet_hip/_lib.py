@@ -38,6 +38,7 @@ class QHObservable(C.Structure):
 
 _ctx_p = C.c_void_p
 _mps_p = C.c_void_p
+_mpo_p = C.c_void_p
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
@@ -78,6 +79,10 @@ SIGNATURES = {
     "qh_expect_many": (C.c_int, [_ctx_p, _mps_p, C.c_int, _ip, C.POINTER(QHObservable), _dp]),
     "qh_overlap": (C.c_int, [_ctx_p, _mps_p, _mps_p, _dp]),
     "qh_norm": (C.c_int, [_ctx_p, _mps_p, _dp]),
+    "qh_mpo_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mpo_p)]),
+    "qh_mpo_free": (C.c_int, [_mpo_p]),
+    "qh_mpo_apply": (C.c_int, [_ctx_p, _mpo_p, _mps_p, C.POINTER(_mps_p)]),
+    "qh_mpo_fit": (C.c_int, [_ctx_p, _mpo_p, _mps_p, _mps_p, C.c_int, C.c_uint, _dp]),
 }
 
 _lib = None
