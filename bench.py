@@ -134,6 +134,74 @@ def batched_leg(qh, wl, precision, count, depth):
     }
 
 
+def _mpo_run(qh, qm, wl, n, chi, D, sweeps):
+    arrays, marr = wl.config4(n=n, chi=chi, D=D)
+    ctx = qh.default_context()
+    psi = qh.from_internal(arrays)
+    h = qm.MPO(marr)
+    z_terms = [[qh.gate(wl.Z, i)] for i in range(1, n + 1)]
+    res = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        qm.compress(h, psi, sweeps=1, precision=prec)  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        phi, ov = qm.compress(h, psi, sweeps=sweeps, precision=prec)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        nrm2 = qh.norm(phi) ** 2
+        z = np.real(qh.expect_many(phi, z_terms)) / nrm2
+        res[prec] = {"seconds": dt, "overlap": ov, "z": z}
+        log(f"mpo fit n={n} chi={chi} {prec}: {dt:.2f} s, overlap {ov.real:.12e}")
+    return arrays, marr, res
+
+
+def _mpo_errors(res, ref_ov, ref_z, sweeps):
+    return {p: {"s_per_sweep": round(r["seconds"] / sweeps, 4),
+                "overlap_rel_err": float(abs(r["overlap"] - ref_ov) / abs(ref_ov)),
+                "max_abs_dZ": float(np.max(np.abs(r["z"] - ref_z)))} for p, r in res.items()}
+
+
+def mpo_leg(qh, wl, n, chi, D, sweeps, cpu, cpu_chi):
+    """MPO application + variational compression back to χ (workload 4).  At the full
+    size each precision is compared with the FP64 GPU path; on a bounded sample
+    (χ = cpu_chi, same n and D) all three are compared with the FP64 CPU oracle,
+    which also gives the CPU baseline (the oracle at χ = 512 takes minutes per sweep)."""
+    from qrochet_hip import mpo as qm
+
+    _, _, res = _mpo_run(qh, qm, wl, n, chi, D, sweeps)
+    ref = res["fp64"]
+    out = {
+        "metric": "MPO application + variational compression: seconds per sweep and error per precision",
+        "value": round(ref["seconds"] / sweeps, 4), "unit": "s/sweep (FP64)", "n_gpus": 1,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"random Hermitian nearest-neighbour MPO (D={D}, complex Gaussian / sqrt(D d), "
+                               f"Hermitian d x d blocks) on a random normalised MPS n={n}, d=2, chi={chi}; one-site "
+                               f"variational fit back to chi={chi} from phi0 = psi, {sweeps} sweeps",
+                   "n": n, "chi": chi, "D": D, "sweeps": sweeps},
+        "precision_vs_fp64_gpu": _mpo_errors(res, ref["overlap"], ref["z"], sweeps),
+        "overlap_fp64": [ref["overlap"].real, ref["overlap"].imag],
+    }
+    if cpu:
+        from oracle import chain as oc
+        from oracle import mpo as om
+
+        arrays, marr, res_s = _mpo_run(qh, qm, wl, n, cpu_chi, D, sweeps)
+        t0 = time.perf_counter()
+        ws = om.from_arrays(marr)
+        psi_ref = oc.Chain([a.copy() for a in arrays])
+        phi_ref = psi_ref.copy()
+        for s in range(sweeps):  # one sweep per call, continuing from the previous φ
+            phi_ref, ovs = om.fit(ws, psi_ref, phi_ref, sweeps=1)
+            log(f"cpu oracle sweep {s} (chi={cpu_chi}): {time.perf_counter() - t0:.1f} s")
+        ct = time.perf_counter() - t0
+        out["precision_vs_cpu_oracle"] = {"chi": cpu_chi, **_mpo_errors(res_s, ovs[-1], np.real(om.local_z(phi_ref)),
+                                                                          sweeps)}
+        out["cpu_baseline"] = {"value": round(ct / sweeps, 3), "unit": "s/sweep", "kind": "port",
+                               "cores": os.cpu_count(),
+                               "sample": f"the same fit at chi={cpu_chi} in numpy (einsum/zgemm), {sweeps} sweeps"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,8 +216,14 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run only the request loop, print its own line")
     ap.add_argument("--serving-requests", type=int, default=None)
     ap.add_argument("--qps", type=float, default=None)
-    ap.add_argument("--workload", choices=("tebd", "batched"), default="tebd",
-                    help="tebd: the n=64, chi=256 Trotter evolution (default); batched: 512 uneven MPS")
+    ap.add_argument("--workload", choices=("tebd", "batched", "mpo"), default="tebd",
+                    help="tebd: the n=64, chi=256 Trotter evolution (default); batched: 512 uneven MPS; "
+                         "mpo: MPO application + variational compression precision study")
+    ap.add_argument("--mpo-n", type=int, default=32)
+    ap.add_argument("--mpo-chi", type=int, default=512)
+    ap.add_argument("--mpo-D", type=int, default=8)
+    ap.add_argument("--mpo-sweeps", type=int, default=2)
+    ap.add_argument("--mpo-cpu-chi", type=int, default=256, help="chi of the sample checked against the CPU oracle")
     ap.add_argument("--precision", choices=("fp64", "fp32", "bf16x3"), default=None,
                     help="contraction precision of the batched workload (default fp32)")
     ap.add_argument("--batch-count", type=int, default=512)
@@ -174,6 +248,13 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 3 launches, each 1 GiB read + 1 GiB written (torch elementwise mul by 1)")
         del src, dst
+
+    if args.workload == "mpo":
+        from qrochet_hip import workloads as wl
+
+        print(json.dumps(mpo_leg(qh, wl, args.mpo_n, args.mpo_chi, args.mpo_D, args.mpo_sweeps, not args.no_cpu,
+                                 args.mpo_cpu_chi)))
+        return
 
     if args.workload == "batched":
         from qrochet_hip import workloads as wl

@@ -18,6 +18,9 @@ import numpy as np
 from . import chain as oc
 
 ORDER = ("o", "i", "l", "r")
+# pairwise contraction order without numpy's default cap on intermediate sizes (the
+# cap makes einsum fall back to the naive 8-index loop for these networks)
+_OPT = ("greedy", 1e13)
 
 
 def from_arrays(arrays, order=ORDER):
@@ -131,13 +134,13 @@ def fit(ws, psi: oc.Chain, phi0: oc.Chain, sweeps=2):
     R[n] = one
 
     def right_env(i):  # R[i-1] from R[i], site i (1-based)
-        return np.einsum("bic,woiv,aox,cxv->baw", A[i - 1], ws[i - 1], B[i - 1].conj(), R[i], optimize=True)
+        return np.einsum("bic,woiv,aox,cxv->baw", A[i - 1], ws[i - 1], B[i - 1].conj(), R[i], optimize=_OPT)
 
     def left_env(i):  # L[i] from L[i-1], site i
-        return np.einsum("baw,bic,woiv,aox->cxv", L[i - 1], A[i - 1], ws[i - 1], B[i - 1].conj(), optimize=True)
+        return np.einsum("baw,bic,woiv,aox->cxv", L[i - 1], A[i - 1], ws[i - 1], B[i - 1].conj(), optimize=_OPT)
 
     def local(i):
-        return np.einsum("baw,bic,woiv,cxv->aox", L[i - 1], A[i - 1], ws[i - 1], R[i], optimize=True)
+        return np.einsum("baw,bic,woiv,cxv->aox", L[i - 1], A[i - 1], ws[i - 1], R[i], optimize=_OPT)
 
     for i in range(n, 1, -1):
         R[i - 1] = right_env(i)
@@ -158,3 +161,23 @@ def fit(ws, psi: oc.Chain, phi0: oc.Chain, sweeps=2):
         B[0] = m
         overlaps.append(complex(np.vdot(m, m)))
     return oc.Chain(B), overlaps
+
+
+def local_z(psi: oc.Chain, op=None):
+    """⟨ψ|O_k|ψ⟩ / ⟨ψ|ψ⟩ for a one-site operator on every site (default Z), from one
+    pair of environment sweeps (the same numbers as `expect` per site,
+    src/Ansatz/Chain.jl:724-735, normalised)."""
+    op = np.diag([1.0, -1.0]).astype(np.complex128) if op is None else op
+    xs = oc._absorbed(psi)
+    n = len(xs)
+    left = [np.ones((1, 1), dtype=np.complex128)]
+    for a in xs:
+        left.append(np.einsum("ab,aoc,bod->cd", left[-1], a.conj(), a, optimize=_OPT))
+    right = [None] * (n + 1)
+    right[n] = np.ones((1, 1), dtype=np.complex128)
+    for k in range(n - 1, -1, -1):
+        a = xs[k]
+        right[k] = np.einsum("cd,aoc,bod->ab", right[k + 1], a.conj(), a, optimize=_OPT)
+    nrm = left[n][0, 0]
+    return np.array([np.einsum("ab,aoc,op,bpd,cd->", left[k], xs[k].conj(), op, xs[k], right[k + 1], optimize=_OPT)
+                     for k in range(n)]) / nrm

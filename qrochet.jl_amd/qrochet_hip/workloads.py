@@ -132,3 +132,32 @@ def config3(seed=3, count=512, depth=10, nmin=8, nmax=64, chis=(16, 32, 64, 128,
     for it in items:
         it["layers"] = [[(b, haar_gate4(grng)) for b in brickwork_bonds(it["n"], layer)] for layer in range(depth)]
     return items
+
+
+# ---------------------------------------------------------------------------
+# workload 4: random Hermitian nearest-neighbour MPO applied to a random MPS
+# ---------------------------------------------------------------------------
+def random_hermitian_mpo(rng, n, d, D):
+    """Reference-shaped MPO arrays (o, i, l, r): complex Gaussian entries × 1/sqrt(D·d),
+    each d×d block W[w, :, :, v] replaced by its Hermitian part (so H is Hermitian)."""
+    dims = [1] + [D] * (n - 1) + [1]
+    out = []
+    for k in range(n):
+        shp = (dims[k], d, d, dims[k + 1])
+        w = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) / np.sqrt(2.0 * D * d)
+        w = 0.5 * (w + np.conj(np.transpose(w, (0, 2, 1, 3))))
+        a = np.transpose(w, (1, 2, 0, 3))  # (o, i, l, r)
+        if k == 0:
+            a = a[:, :, 0, :]
+        if k == n - 1:
+            a = a[..., 0]
+        out.append(np.ascontiguousarray(a))
+    return out
+
+
+def config4(seed=4, n=32, d=2, chi=512, D=8):
+    """Random normalised MPS at the χ profile and a random Hermitian MPO of bond D."""
+    rng = np.random.default_rng(seed)
+    arrays = left_canonize_normalize(random_mps_arrays(rng, n, d, max_profile(n, d, chi)))
+    mpo = random_hermitian_mpo(rng, n, d, D)
+    return arrays, mpo

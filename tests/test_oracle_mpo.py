@@ -99,3 +99,13 @@ def test_fit_is_variational():
     assert all(ov[k + 1].real >= ov[k].real - 1e-12 for k in range(len(ov) - 1))
     # compression loses something here (χ = 4 cannot hold H|ψ⟩ exactly)
     assert np.vdot(hv, hv).real - ov[-1].real > 1e-6
+
+
+def test_local_z_matches_expect():
+    rng = np.random.default_rng(7)
+    n = 6
+    psi = _psi(rng, n, 4, canon=True)
+    z = om.local_z(psi)
+    nrm = oc.norm(psi) ** 2
+    ref = np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]) / nrm
+    assert np.max(np.abs(z - ref)) < 1e-12

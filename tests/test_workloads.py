@@ -43,3 +43,11 @@ def test_config3_inputs_identical():
         assert (x["n"], x["chi"]) == (y["n"], y["chi"])
         _same_arrays(x["arrays"], y["arrays"])
         _same_layers(x["layers"], y["layers"])
+
+
+def test_mpo_generator_identical():
+    from oracle import mpo as om
+
+    a = om.random_hermitian_mpo(np.random.default_rng(9), 5, 2, 3)
+    b = pw.random_hermitian_mpo(np.random.default_rng(9), 5, 2, 3)
+    _same_arrays(a, b)
