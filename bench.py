@@ -318,7 +318,14 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(frac_m, 4)}
+    # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
     roof["traffic"] = None
+    tf = os.path.join(ROOT, "profiles", "k_jacobi_round_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        roof["traffic"] = t["traffic_bytes_per_launch"]
+        roof["traffic_source"] = f"profiles/k_jacobi_round_traffic.json ({t['workload']})"
     roof["kernel"] = "k_jacobi_round<32> (block one-sided Jacobi round, FP64)"
     roof["avg_launch_ms"] = round(avg_ms, 5)
     roof["launches"] = jr["launches"]
