@@ -95,6 +95,17 @@ int qh_ctx_mem_info(qh_context* ctx, size_t* free_bytes, size_t* total_bytes);
 int qh_mps_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays,
                   const double* const* lambdas, qh_mps** out);
 /* `copy`/`deepcopy` (src/Ansatz/Chain.jl:11): independent device copy. */
+/* Replaces `pMPS(arrays)` = `Chain(State(), Periodic(), arrays)` (src/Ansatz/Chain.jl:19,36-62).
+ * Every array is (o, l, r) column-major (3-D); bond_dims has n entries, bond i joins
+ * sites i and i+1 and bond n joins site n to site 1.  Periodic chains support
+ * evolve (bonds 1..n-1, as the reference: `Gate lanes must be contiguous`),
+ * truncate, contract_between, the canonicality tests, expect, overlap and norm
+ * (trace over the wrap bond, cost O(n·W²·χ³) for wrap dimension W); the methods the
+ * reference defines for `Open` only return QH_ERR_ARGUMENT. */
+int qh_pmps_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays, qh_mps** out);
+/* periodic: 1 for a periodic chain, 0 for an open one; wrap_dim: the bond between
+ * site n and site 1 (1 for an open chain) */
+int qh_mps_is_periodic(const qh_mps* mps, int* periodic, int* wrap_dim);
 int qh_mps_clone(qh_context* ctx, const qh_mps* src, qh_mps** out);
 int qh_mps_free(qh_mps* mps);
 int qh_mps_nsites(const qh_mps* mps, int* n, int* d);

@@ -111,6 +111,9 @@ def gate1(matrix, i: int) -> Dense:
 class Chain:
     arrays: list
     lam: list = field(default_factory=list)
+    # Periodic boundary (`pMPS`, `Chain.jl:19,36-62`): site 1's left bond is site n's
+    # right bond (dimension arrays[0].shape[0]); Λ vectors live on bonds 1..n-1 only
+    periodic: bool = False
 
     def __post_init__(self):
         if not self.lam:
@@ -129,7 +132,8 @@ class Chain:
         return [a.shape[2] for a in self.arrays[:-1]]
 
     def copy(self) -> "Chain":
-        return Chain([a.copy() for a in self.arrays], [None if l is None else l.copy() for l in self.lam])
+        return Chain([a.copy() for a in self.arrays], [None if l is None else l.copy() for l in self.lam],
+                     self.periodic)
 
     def ntensors(self) -> int:
         """Number of tensors in the reference TN (sites + Λ vectors)."""
@@ -172,6 +176,19 @@ def MPS(arrays, order=("o", "l", "r")) -> Chain:
     return Chain(out)
 
 
+def pMPS(arrays, order=("o", "l", "r")) -> Chain:
+    """`pMPS(arrays)` / `Chain(::State, ::Periodic, arrays; order)` (`Chain.jl:19,36-62`):
+    every array 3-D; site n's right bond is site 1's left bond."""
+    assert all(np.ndim(a) == 3 for a in arrays), "All arrays must have 3 dimensions"
+    if sorted(order) != ["l", "o", "r"]:
+        raise ArgumentError("order must be a permutation of (o, l, r)")
+    out = [np.ascontiguousarray(np.transpose(np.asarray(a, dtype=np.complex128),
+                                             [list(order).index(x) for x in ("l", "o", "r")])) for a in arrays]
+    for i in range(len(out)):
+        assert out[i].shape[2] == out[(i + 1) % len(out)].shape[0], "bond dimension mismatch"
+    return Chain(out, periodic=True)
+
+
 def to_arrays(psi: Chain, order=("o", "l", "r")):
     """Inverse of `MPS`: reference-shaped arrays (boundary tensors 2-D)."""
     res = []
@@ -191,6 +208,13 @@ def to_arrays(psi: Chain, order=("o", "l", "r")):
             names = names[:-1]
         res.append(np.transpose(t, [names.index(x) for x in o]))
     return res
+
+
+def _open_only(psi: Chain):
+    """canonize_site!/canonize!/mixed_canonize! dispatch on `::Open` only
+    (`Chain.jl:336,339,461,469,500,509`)."""
+    if psi.periodic:
+        raise ArgumentError("defined for open chains only")
 
 
 def _check_site(psi: Chain, site: int):
@@ -259,6 +283,7 @@ def canonize_site(psi: Chain, site: int, direction: str, method: str = "qr") -> 
     bond) is contracted -- together with any Λ already on that bond -- into
     the neighbouring tensor.
     """
+    _open_only(psi)
     _check_site(psi, site)
     i = site - 1
     if direction == "left":
@@ -382,6 +407,7 @@ def isrightcanonical(psi: Chain, site: int, atol: float = 1e-12) -> bool:
 # ---------------------------------------------------------------------------
 def canonize(psi: Chain) -> Chain:
     """`canonize!(::Open, tn)` (`Chain.jl:469-497`): Vidal form Γ₁Λ₁Γ₂…Γₙ."""
+    _open_only(psi)
     n = psi.n
     for i in range(n, 1, -1):
         canonize_site(psi, i, "left", "qr")
@@ -401,6 +427,7 @@ def canonize(psi: Chain) -> Chain:
 
 def mixed_canonize(psi: Chain, center: int) -> Chain:
     """`mixed_canonize!(::Open, tn, center)` (`Chain.jl:509-524`)."""
+    _open_only(psi)
     for i in range(1, center):
         canonize_site(psi, i, "right", "qr")
     for i in range(psi.n, center, -1):
@@ -528,6 +555,14 @@ def overlap(a: Chain, b: Chain) -> complex:
     if a.n != b.n:
         raise AssertionError("Ansatzes must have the same sites")
     xa, xb = _absorbed(a), _absorbed(b)
+    if a.periodic or b.periodic:
+        # carry the two wrap indices (bra p, ket q) through the chain, then trace
+        wb, wa = xb[0].shape[0], xa[0].shape[0]
+        e = np.einsum("pl,qm->pqlm", np.eye(wb), np.eye(wa)).astype(np.complex128)
+        for ta, tb in zip(xa, xb):
+            t = np.tensordot(e, ta, axes=(3, 0))  # (p, q, l_b, o, r_a)
+            e = np.einsum("lor,pqlos->pqrs", tb.conj(), t)
+        return complex(np.einsum("pqpq->", e))
     e = np.ones((1, 1), dtype=np.complex128)  # e[l_b, l_a]
     for ta, tb in zip(xa, xb):
         t = np.tensordot(e, ta, axes=(1, 0))  # (l_b, o, r_a)
@@ -552,8 +587,13 @@ def expect(psi: Chain, observables) -> complex:
 
 def statevector(psi: Chain) -> np.ndarray:
     """Full contraction incl. Λ's (`transform(tn, HyperFlatten())` + `contract`).
-    Axis k is site k+1."""
+    Axis k is site k+1.  Periodic chains trace over the wrap bond."""
     xs = _absorbed(psi)
+    if psi.periodic:
+        v = xs[0]  # (w, o, r)
+        for t in xs[1:]:
+            v = np.tensordot(v, t, axes=(v.ndim - 1, 0))
+        return np.trace(v, axis1=0, axis2=v.ndim - 1)
     v = xs[0][0]  # (o, r)
     for t in xs[1:]:
         v = np.tensordot(v, t, axes=(v.ndim - 1, 0))

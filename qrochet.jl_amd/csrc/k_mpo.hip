@@ -66,4 +66,19 @@ void launch_dot(const double2* x, const double2* y, size_t n, double2* out, hipS
     k_dot<<<1, 256, 0, s>>>(x, y, n, out);
 }
 
+// *out = Σ_p e[p + n p]  (trace of an n × n matrix, one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_trace(const double2* __restrict__ e, int n, double2* __restrict__ out) {
+    __shared__ double2 sRed[256];
+    double2 s = cz();
+    for (int p = threadIdx.x; p < n; p += 256) s = cadd(s, e[(size_t)p * (n + 1)]);
+    sRed[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sRed[threadIdx.x] = cadd(sRed[threadIdx.x], sRed[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sRed[0];
+}
+void launch_trace(const double2* e, int n, double2* out, hipStream_t s) { k_trace<<<1, 256, 0, s>>>(e, n, out); }
+
 }  // namespace qh

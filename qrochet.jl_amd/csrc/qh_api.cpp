@@ -17,6 +17,8 @@ using namespace qh;
 
 namespace qh {
 void launch_apply_op(cplx* t, int left, int dim, int right, const cplx* op, hipStream_t s);
+void launch_eye_rect(cplx* p, int m, int n, hipStream_t s);
+void launch_trace(const cplx* e, int n, cplx* out, hipStream_t s);
 }
 
 namespace {
@@ -35,6 +37,12 @@ void check_site(const qh_mps* m, int site) {
 void check_bond(const qh_mps* m, int bond) {
     if (bond < 1 || bond > m->n - 1) throw Error(QH_ERR_ARGUMENT, "Invalid bond (" + std::to_string(bond) + ", " +
                                                                        std::to_string(bond + 1) + ")");
+}
+
+// canonize_site!/canonize!/mixed_canonize! (and normalize!) dispatch on `::Open`
+// only (src/Ansatz/Chain.jl:336,339,461,469,500,509)
+void check_open(const qh_mps* m, const char* what) {
+    if (m->periodic) throw Error(QH_ERR_ARGUMENT, std::string(what) + " is defined for open chains only");
 }
 
 cplx* dcopy(qh_context* c, const cplx* src, size_t n) {
@@ -151,6 +159,7 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
 // factorisations of one site (Chain.jl:339-376)
 // ---------------------------------------------------------------------------
 void canonize_site_core(qh_context* c, qh_mps* m, int site, int direction, int method) {
+    check_open(m, "canonize_site!");
     check_site(m, site);
     if (direction == QH_LEFT) {
         if (site == 1) throw Error(QH_ERR_ARGUMENT, "Cannot right-canonize left-most tensor");
@@ -299,6 +308,7 @@ void contract_between_core(qh_context* c, qh_mps* m, int s1, int s2, int directi
 
 // Chain.jl:469-497
 void canonize_core(qh_context* c, qh_mps* m) {
+    check_open(m, "canonize!");
     const int n = m->n;
     for (int i = n; i >= 2; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
     std::vector<double*> lams(n - 1, nullptr);
@@ -317,6 +327,7 @@ void canonize_core(qh_context* c, qh_mps* m) {
 
 // Chain.jl:509-524
 void mixed_canonize_core(qh_context* c, qh_mps* m, int center) {
+    check_open(m, "mixed_canonize!");
     check_site(m, center);
     for (int i = 1; i <= center - 1; ++i) canonize_site_core(c, m, i, QH_RIGHT, QH_METHOD_QR);
     for (int i = m->n; i >= center + 1; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
@@ -496,7 +507,91 @@ cplx* unit_env(qh_context* c) {
     return e;
 }
 
+// Periodic chains: the wrap bonds of bra (Wb) and ket (Wk) ride along as a passenger
+// index p = (wb, wk), P = Wb·Wk; the environment E[b, a, p] starts as the identity
+// E[(b, a), p] = δ((b, a), p) and every step is the open-chain step batched over p.
+// The value is the trace Σ_p E[(b, a) = p, p] after the last site.  Cost
+// O(n · Wb·Wk · χ³): meant for moderate bond dimensions.
+zc periodic_contraction(qh_context* c, const qh_mps* ket, const qh_mps* bra, const std::vector<Cluster>& cls) {
+    const int d = ket->d, n = ket->n;
+    const int Wk = ket->chi[0], Wb = bra->chi[0];
+    const int P = Wk * Wb;
+    cplx* E = dalloc<cplx>(c, (size_t)P * P);
+    launch_eye_rect(E, P, P, c->stream);
+    size_t ci = 0;
+    int i = 1;
+    while (i <= n) {
+        const int lb = bra->chi[i - 1], la = ket->chi[i - 1];
+        const bool has_cluster = ci < cls.size() && cls[ci].start == i;
+        const int w = has_cluster ? cls[ci].width : 1;
+        const int ra1 = ket->chi[i];
+        cplx* T = dalloc<cplx>(c, (size_t)lb * d * ra1 * P);
+        {
+            std::vector<GemmItem> g;
+            for (int p = 0; p < P; ++p) {
+                g.push_back(gemm_item(E + (size_t)lb * la * p, 0, lb, ket->site[i - 1], 0, la,
+                                      T + (size_t)lb * d * ra1 * p, lb, lb, d * ra1, la));
+                if (i >= 2) { g.back().dk = ket->lam[i - 2]; g.back().dk_mod = la; }
+            }
+            run_gemm(c, g);
+        }
+        dfree(c, E);
+        for (int s2 = 1; s2 < w; ++s2) {
+            const int j = i + s2;
+            const int rows = lb * ipow(d, s2);
+            const int kj = ket->chi[j - 1], rj = ket->chi[j];
+            cplx* T2 = dalloc<cplx>(c, (size_t)rows * d * rj * P);
+            std::vector<GemmItem> g;
+            for (int p = 0; p < P; ++p) {
+                g.push_back(gemm_item(T + (size_t)rows * kj * p, 0, rows, ket->site[j - 1], 0, kj,
+                                      T2 + (size_t)rows * d * rj * p, rows, rows, d * rj, kj));
+                g.back().dk = ket->lam[j - 2];
+                g.back().dk_mod = kj;
+            }
+            run_gemm(c, g);
+            dfree(c, T);
+            T = T2;
+        }
+        const int ra = ket->chi[i + w - 1];
+        if (has_cluster) {
+            std::vector<cplx> hop(cls[ci].op.size());
+            std::memcpy(hop.data(), cls[ci].op.data(), hop.size() * sizeof(cplx));
+            const cplx* dop = upload(c, hop);
+            launch_apply_op(T, lb, ipow(d, w), ra * P, dop, c->stream);
+        }
+        int curl = lb;
+        for (int s2 = 0; s2 < w; ++s2) {
+            const int j = i + s2;
+            const int rest = ipow(d, w - 1 - s2) * ra;
+            const int rb = bra->chi[j];
+            cplx* Wm = dalloc<cplx>(c, (size_t)rb * rest * P);
+            std::vector<GemmItem> g;
+            for (int p = 0; p < P; ++p) {
+                g.push_back(gemm_item(bra->site[j - 1], 1, curl * d, T + (size_t)curl * d * rest * p, 0, curl * d,
+                                      Wm + (size_t)rb * rest * p, rb, rb, rest, curl * d));
+                if (j >= 2) { g.back().dk = bra->lam[j - 2]; g.back().dk_mod = curl; }
+            }
+            run_gemm(c, g);
+            dfree(c, T);
+            T = Wm;
+            curl = rb;
+        }
+        E = T;
+        i += w;
+        if (has_cluster) ++ci;
+    }
+    cplx* tr = dalloc<cplx>(c, 1);
+    launch_trace(E, P, tr, c->stream);
+    cplx h;
+    readback(c, &h, tr, sizeof(cplx));
+    dfree(c, tr);
+    dfree(c, E);
+    return zc(h.x, h.y);
+}
+
 zc full_contraction(qh_context* c, const qh_mps* ket, const qh_mps* bra, const std::vector<Cluster>& cls) {
+    if (ket->periodic != bra->periodic) throw Error(QH_ERR_ARGUMENT, "cannot contract a periodic with an open chain");
+    if (ket->periodic) return periodic_contraction(c, ket, bra, cls);
     cplx* E = env_segment(c, ket, bra, unit_env(c), 1, ket->n, cls);
     cplx h;
     readback(c, &h, E, sizeof(cplx));
@@ -536,6 +631,70 @@ std::vector<cplx*> left_envs(qh_context* c, const qh_mps* ket, const qh_mps* bra
     return L;
 }
 
+}  // namespace
+
+namespace {
+// `Chain(::State, boundary, arrays)` (src/Ansatz/Chain.jl:36-62 periodic, :64-100 open)
+int mps_create_impl(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays,
+                    const double* const* lambdas, bool periodic, qh_mps** out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    qh_mps* m = nullptr;
+    int rc = guard(c, [&] {
+        if (n < 2) throw Error(QH_ERR_ARGUMENT, "a Chain needs at least 2 sites");
+        if (d < 2 || d > 4) throw Error(QH_ERR_ARGUMENT, "physical dimension must be 2, 3 or 4");
+        if (!bond_dims || !arrays) throw Error(QH_ERR_ARGUMENT, "null argument");
+        const int nb = periodic ? n : n - 1;
+        for (int b = 0; b < nb; ++b)
+            if (bond_dims[b] < 1) throw Error(QH_ERR_DIMENSION, "bond dimensions must be positive");
+        m = new qh_mps();
+        m->ctx = c;
+        m->n = n;
+        m->d = d;
+        m->periodic = periodic;
+        m->chi.assign(n + 1, 1);
+        for (int b = 1; b <= n - 1; ++b) m->chi[b] = bond_dims[b - 1];
+        if (periodic) m->chi[0] = m->chi[n] = bond_dims[n - 1];
+        m->site.assign(n, nullptr);
+        m->lam.assign(n - 1, nullptr);
+        std::vector<std::vector<cplx>> stage(n);
+        for (int i = 1; i <= n; ++i) {
+            if (!arrays[i - 1]) throw Error(QH_ERR_ARGUMENT, "null site array");
+            const int cl = m->chi[i - 1], cr = m->chi[i];
+            const size_t ne = site_elems(m, i);
+            const cplx* src = reinterpret_cast<const cplx*>(arrays[i - 1]);
+            auto& st = stage[i - 1];
+            st.resize(ne);
+            // (o, l, r) -> (l, o, r)
+            for (int r = 0; r < cr; ++r)
+                for (int l = 0; l < cl; ++l)
+                    for (int o = 0; o < d; ++o)
+                        st[l + (size_t)cl * (o + (size_t)d * r)] = src[o + (size_t)d * (l + (size_t)cl * r)];
+            m->site[i - 1] = dalloc<cplx>(c, ne);
+            keep(c, m->site[i - 1]);
+            QH_HIP(hipMemcpyAsync(m->site[i - 1], st.data(), ne * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+        }
+        if (lambdas) {
+            for (int b = 1; b <= n - 1; ++b) {
+                if (!lambdas[b - 1]) continue;
+                m->lam[b - 1] = dalloc<double>(c, m->chi[b]);
+                keep(c, m->lam[b - 1]);
+                QH_HIP(hipMemcpyAsync(m->lam[b - 1], lambdas[b - 1], m->chi[b] * sizeof(double), hipMemcpyHostToDevice,
+                                      c->stream));
+            }
+        }
+        ctx_sync(c);
+    });
+    if (rc != QH_OK) {
+        if (m) {
+            for (auto p : m->site) dfree(c, p);
+            for (auto p : m->lam) dfree(c, p);
+            delete m;
+        }
+        return rc;
+    }
+    *out = m;
+    return QH_OK;
+}
 }  // namespace
 
 // ===========================================================================
@@ -644,59 +803,17 @@ int qh_ctx_mem_info(qh_context* c, size_t* free_bytes, size_t* total_bytes) {
 
 int qh_mps_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays,
                   const double* const* lambdas, qh_mps** out) {
-    if (!c || !out) return QH_ERR_ARGUMENT;
-    qh_mps* m = nullptr;
-    int rc = guard(c, [&] {
-        if (n < 2) throw Error(QH_ERR_ARGUMENT, "a Chain needs at least 2 sites");
-        if (d < 2 || d > 4) throw Error(QH_ERR_ARGUMENT, "physical dimension must be 2, 3 or 4");
-        if (!bond_dims || !arrays) throw Error(QH_ERR_ARGUMENT, "null argument");
-        for (int b = 0; b < n - 1; ++b)
-            if (bond_dims[b] < 1) throw Error(QH_ERR_DIMENSION, "bond dimensions must be positive");
-        m = new qh_mps();
-        m->ctx = c;
-        m->n = n;
-        m->d = d;
-        m->chi.assign(n + 1, 1);
-        for (int b = 1; b <= n - 1; ++b) m->chi[b] = bond_dims[b - 1];
-        m->site.assign(n, nullptr);
-        m->lam.assign(n - 1, nullptr);
-        std::vector<std::vector<cplx>> stage(n);
-        for (int i = 1; i <= n; ++i) {
-            if (!arrays[i - 1]) throw Error(QH_ERR_ARGUMENT, "null site array");
-            const int cl = m->chi[i - 1], cr = m->chi[i];
-            const size_t ne = site_elems(m, i);
-            const cplx* src = reinterpret_cast<const cplx*>(arrays[i - 1]);
-            auto& st = stage[i - 1];
-            st.resize(ne);
-            // (o, l, r) -> (l, o, r)
-            for (int r = 0; r < cr; ++r)
-                for (int l = 0; l < cl; ++l)
-                    for (int o = 0; o < d; ++o)
-                        st[l + (size_t)cl * (o + (size_t)d * r)] = src[o + (size_t)d * (l + (size_t)cl * r)];
-            m->site[i - 1] = dalloc<cplx>(c, ne);
-            keep(c, m->site[i - 1]);
-            QH_HIP(hipMemcpyAsync(m->site[i - 1], st.data(), ne * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
-        }
-        if (lambdas) {
-            for (int b = 1; b <= n - 1; ++b) {
-                if (!lambdas[b - 1]) continue;
-                m->lam[b - 1] = dalloc<double>(c, m->chi[b]);
-                keep(c, m->lam[b - 1]);
-                QH_HIP(hipMemcpyAsync(m->lam[b - 1], lambdas[b - 1], m->chi[b] * sizeof(double), hipMemcpyHostToDevice,
-                                      c->stream));
-            }
-        }
-        ctx_sync(c);
-    });
-    if (rc != QH_OK) {
-        if (m) {
-            for (auto p : m->site) dfree(c, p);
-            for (auto p : m->lam) dfree(c, p);
-            delete m;
-        }
-        return rc;
-    }
-    *out = m;
+    return mps_create_impl(c, n, d, bond_dims, arrays, lambdas, false, out);
+}
+
+int qh_pmps_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays, qh_mps** out) {
+    return mps_create_impl(c, n, d, bond_dims, arrays, nullptr, true, out);
+}
+
+int qh_mps_is_periodic(const qh_mps* m, int* periodic, int* wrap_dim) {
+    if (!m || !periodic || !wrap_dim) return QH_ERR_ARGUMENT;
+    *periodic = m->periodic ? 1 : 0;
+    *wrap_dim = m->chi[0];
     return QH_OK;
 }
 
@@ -907,6 +1024,14 @@ int qh_expect_many(qh_context* c, qh_mps* m, int nterms, const int* offsets, con
         for (int t = 0; t < nterms; ++t) {
             if (offsets[t + 1] < offsets[t]) throw Error(QH_ERR_ARGUMENT, "bad term offsets");
             terms[t] = make_clusters(m, offsets[t + 1] - offsets[t], obs + offsets[t]);
+        }
+        if (m->periodic) {  // no shared environments: one periodic contraction per term
+            for (int t = 0; t < nterms; ++t) {
+                const zc v = periodic_contraction(c, m, m, terms[t]);
+                out[2 * t] = v.real();
+                out[2 * t + 1] = v.imag();
+            }
+            return;
         }
         auto L = left_envs(c, m, m);
         auto R = right_envs(c, m, m);

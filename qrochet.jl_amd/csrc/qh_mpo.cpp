@@ -215,6 +215,7 @@ void mpo_fit_core(qh_context* c, const qh_mpo* h, const qh_mps* psi, qh_mps* phi
     check_mps(c, psi);
     check_mps(c, phi);
     if (psi == phi) throw Error(QH_ERR_ARGUMENT, "the fitted MPS must be distinct from the input MPS");
+    if (psi->periodic || phi->periodic) throw Error(QH_ERR_ARGUMENT, "MPO fitting is defined for open chains only");
     if (psi->n != h->n || phi->n != h->n) throw Error(QH_ERR_DIMENSION, "MPO and MPS lengths differ");
     if (psi->d != h->d || phi->d != h->d) throw Error(QH_ERR_DIMENSION, "physical dimensions differ");
     if (sweeps < 1) throw Error(QH_ERR_ARGUMENT, "sweeps must be positive");
@@ -378,6 +379,7 @@ int qh_mpo_apply(qh_context* c, const qh_mpo* h, const qh_mps* psi, qh_mps** out
         check_mps(c, psi);
         if (psi->n != h->n) throw Error(QH_ERR_DIMENSION, "MPO and MPS lengths differ");
         if (psi->d != h->d) throw Error(QH_ERR_DIMENSION, "physical dimensions differ");
+        if (psi->periodic) throw Error(QH_ERR_ARGUMENT, "the MPO is open; the MPS is periodic");
         const int n = h->n, d = h->d;
         m = new qh_mps();
         m->ctx = c;

@@ -90,7 +90,9 @@ struct qh_context {
 struct qh_mps {
     qh_context* ctx = nullptr;
     int n = 0, d = 0;
-    std::vector<int> chi;            // n+1 entries: chi[0] = chi[n] = 1, chi[b] bond b
+    std::vector<int> chi;            // n+1 entries: chi[b] bond b; open: chi[0] = chi[n] = 1,
+                                     // periodic: chi[0] = chi[n] = the wrap bond (site n -> site 1)
+    bool periodic = false;
     std::vector<qh::cplx*> site;     // n
     std::vector<double*> lam;        // n-1 (bond b -> lam[b-1]); null when absent
 };

@@ -31,6 +31,7 @@ from .chain import (  # noqa: F401
     norm,
     normalize_,
     overlap,
+    pMPS,
     simple_update_,
     site,
     truncate,

@@ -54,6 +54,8 @@ SIGNATURES = {
     "qh_ctx_set_jacobi": (C.c_int, [_ctx_p, C.c_int, C.c_int]),
     "qh_ctx_mem_info": (C.c_int, [_ctx_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "qh_mps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_mps_p)]),
+    "qh_pmps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mps_p)]),
+    "qh_mps_is_periodic": (C.c_int, [_mps_p, _ip, _ip]),
     "qh_mps_clone": (C.c_int, [_ctx_p, _mps_p, C.POINTER(_mps_p)]),
     "qh_mps_free": (C.c_int, [_mps_p]),
     "qh_mps_nsites": (C.c_int, [_mps_p, _ip, _ip]),

@@ -235,6 +235,11 @@ class Chain:
         lib.qh_mps_nsites(handle, C.byref(n), C.byref(d))
         self.n = n.value
         self.d = d.value
+        per = C.c_int()
+        wrap = C.c_int()
+        lib.qh_mps_is_periodic(handle, C.byref(per), C.byref(wrap))
+        self.periodic = bool(per.value)
+        self._wrap = wrap.value
         self._fin = weakref.finalize(self, lib.qh_mps_free, handle)
 
     # --- queries -----------------------------------------------------------
@@ -267,19 +272,26 @@ class Chain:
         L.load().qh_mps_footprint(self.handle, C.byref(b))
         return b.value
 
+    def wrap_dim(self):
+        """Dimension of the bond between site n and site 1 (1 for an open chain)."""
+        return self._wrap
+
+    def _site_shape(self, i):
+        full = [self._wrap] + self.bond_dims() + [self._wrap]
+        return self.d, full[i - 1], full[i]
+
     def site_array(self, i, order=("o", "l", "r")):
         """Download the tensor at site ``i`` in the reference's shape/order."""
         i = _sid(i)
-        dims = [1] + self.bond_dims() + [1]
-        cl, cr = dims[i - 1], dims[i]
+        _, cl, cr = self._site_shape(i)
         buf = np.empty(self.d * cl * cr, dtype=np.complex128)
         self.ctx.check(L.load().qh_mps_get_site(self.handle, i, buf.ctypes.data_as(C.POINTER(C.c_double))))
         t = buf.reshape((self.d, cl, cr), order="F")
         names = ["o", "l", "r"]
-        if i == 1:
+        if i == 1 and not self.periodic:
             t = t[:, 0, :]
             names = ["o", "r"]
-        if i == self.n:
+        if i == self.n and not self.periodic:
             t = t[..., 0]
             names = names[:-1]
         o = [x for x in order if x in names]
@@ -293,9 +305,9 @@ class Chain:
         out = []
         for i in range(1, self.n + 1):
             a = self.site_array(i)
-            if i == 1:
+            if i == 1 and not self.periodic:
                 a = a[:, None, :]
-            if i == self.n:
+            if i == self.n and not self.periodic:
                 a = a[..., None]
             out.append(np.transpose(a, (1, 0, 2)))
         return out
@@ -361,6 +373,32 @@ def MPS(arrays, order=("o", "l", "r"), lambdas=None, context: Context | None = N
                 lam_ptrs[b] = v.ctypes.data_as(L._dp)
     h = C.c_void_p()
     ctx.check(L.load().qh_mps_create(ctx.handle, n, d, dims, ptrs, lam_ptrs, C.byref(h)))
+    return Chain(h, ctx)
+
+
+def pMPS(arrays, order=("o", "l", "r"), context: Context | None = None) -> Chain:
+    """`pMPS(arrays)` = `Chain(State(), Periodic(), arrays; order)` (src/Ansatz/Chain.jl:19,36-62):
+    every array 3-D; site n's right bond is site 1's left bond."""
+    ctx = context or default_context()
+    n = len(arrays)
+    if n < 2:
+        raise ArgumentError("a Chain needs at least 2 sites")
+    if sorted(order) != ["l", "o", "r"]:
+        raise ArgumentError("order must be a permutation of (o, l, r)")
+    if not all(np.ndim(a) == 3 for a in arrays):
+        raise AssertionError("All arrays must have 3 dimensions")
+    full = [np.asfortranarray(np.transpose(np.asarray(a, dtype=np.complex128),
+                                           [list(order).index(x) for x in ("o", "l", "r")])) for a in arrays]
+    d = full[0].shape[0]
+    for i in range(n):
+        if full[i].shape[0] != d:
+            raise DimensionMismatch("all sites must share the physical dimension")
+        if full[i].shape[2] != full[(i + 1) % n].shape[1]:
+            raise DimensionMismatch(f"bond {i + 1}: {full[i].shape[2]} != {full[(i + 1) % n].shape[1]}")
+    dims = (C.c_int * n)(*[full[i].shape[2] for i in range(n)])
+    ptrs = (L._dp * n)(*[f.ctypes.data_as(L._dp) for f in full])
+    h = C.c_void_p()
+    ctx.check(L.load().qh_pmps_create(ctx.handle, n, d, dims, ptrs, C.byref(h)))
     return Chain(h, ctx)
 
 

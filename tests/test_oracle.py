@@ -395,3 +395,54 @@ def test_golden_statevector_fixture():
         v = sv.apply2(v, fx[f"gate{k}"], int(fx["bonds"][k]))
     assert np.max(np.abs(oc.statevector(psi) - fx["final_state"])) < 1e-12
     assert np.max(np.abs(v - fx["final_state"])) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# periodic chains (pMPS, Chain.jl:19,36-62)
+# ---------------------------------------------------------------------------
+def _ghz_pmps(n):
+    a = np.zeros((2, 2, 2), dtype=np.complex128)  # (l, o, r): A_0 = diag(1,0), A_1 = diag(0,1)
+    a[0, 0, 0] = 1.0
+    a[1, 1, 1] = 1.0
+    return oc.Chain([a.copy() for _ in range(n)], periodic=True)
+
+
+def test_periodic_ghz_closed_form():
+    n = 5
+    psi = _ghz_pmps(n)
+    v = oc.statevector(psi)
+    exp = np.zeros((2,) * n, dtype=np.complex128)
+    exp[(0,) * n] = 1.0
+    exp[(1,) * n] = 1.0
+    assert np.array_equal(v, exp)
+    assert abs(oc.overlap(psi, psi) - 2.0) < 1e-14
+    assert abs(oc.norm(psi) - np.sqrt(2.0)) < 1e-14
+    assert abs(oc.expect(psi, [oc.gate1(wl.Z, 2)])) < 1e-14
+    assert abs(oc.expect(psi, [oc.gate1(wl.Z, 1), oc.gate1(wl.Z, 4)]) - 2.0) < 1e-14
+
+
+def test_periodic_with_unit_wrap_equals_open():
+    rng = np.random.default_rng(3)
+    arrays = wl.random_mps_arrays(rng, 6, 2, wl.max_profile(6, 2, 4))
+    open_ = oc.Chain([a.copy() for a in arrays])
+    per = oc.Chain([a.copy() for a in arrays], periodic=True)
+    assert np.allclose(oc.statevector(per), oc.statevector(open_), atol=1e-14)
+    ref = oc.overlap(open_, open_)
+    assert abs(oc.overlap(per, open_) - ref) < 1e-12 * abs(ref)
+
+
+def test_periodic_evolve_matches_statevector_and_overlap():
+    rng = np.random.default_rng(4)
+    n, w = 6, 3
+    arrays = [(rng.standard_normal((w, 2, w)) + 1j * rng.standard_normal((w, 2, w))) / np.sqrt(2) for _ in range(n)]
+    psi = oc.pMPS([np.transpose(a, (1, 0, 2)) for a in arrays])  # reference order (o, l, r)
+    v = oc.statevector(psi)
+    for layer in range(3):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            oc.evolve(psi, oc.gate2(g, b))
+            v = sv.apply2(v, g, b)
+    assert np.max(np.abs(oc.statevector(psi) - v)) < 1e-12 * np.max(np.abs(v))
+    assert abs(oc.overlap(psi, psi) - np.vdot(v, v)) < 1e-12 * abs(np.vdot(v, v))
+    with pytest.raises(oc.ArgumentError):
+        oc.canonize(psi)
