@@ -185,6 +185,10 @@ int qh_norm(qh_context* ctx, qh_mps* mps, double* out);
  * reference's default order, Chain.jl:34), boundary tensors 3-D: (o, i, r) for the
  * first site, (o, i, l) for the last.  bond_dims: n-1 entries. */
 int qh_mpo_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out);
+/* Replaces `pMPO(arrays)` = `Chain(Operator(), Periodic(), arrays)` (Chain.jl:21,102-131):
+ * every array (o, i, l, r) 4-D, bond_dims n entries (bond n joins site n to site 1).
+ * A periodic MPO applies (exactly) to a periodic MPS; fitting is open-only. */
+int qh_pmpo_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out);
 int qh_mpo_free(qh_mpo* mpo);
 /* H|ψ⟩ exactly: a new MPS with bond dimensions χ_b·D_b (Λ vectors of ψ absorbed).
  * The reference has no apply routine: it would `merge` the MPO and MPS networks

@@ -79,13 +79,20 @@ def random_hermitian_mpo(rng, n, d, D):
 
 
 def dense(ws):
-    """H as a tensor with axes (o_1 … o_n, i_1 … i_n), site k on axes k-1 and n+k-1."""
+    """H as a tensor with axes (o_1 … o_n, i_1 … i_n), site k on axes k-1 and n+k-1.
+    A periodic MPO (first left bond > 1, `pMPO`) is traced over its wrap bond."""
     n = len(ws)
-    t = ws[0][0]  # (o, i, r)
+    t = ws[0]  # (w, o, i, r)
     for w in ws[1:]:
-        t = np.tensordot(t, w, axes=(t.ndim - 1, 0))  # (…, o, i, r')
-    t = t[..., 0]  # (o1, i1, o2, i2, …, on, in)
+        t = np.tensordot(t, w, axes=(t.ndim - 1, 0))  # (w, …, o, i, r')
+    t = np.trace(t, axis1=0, axis2=t.ndim - 1)  # (o1, i1, o2, i2, …, on, in)
     return np.transpose(t, list(range(0, 2 * n, 2)) + list(range(1, 2 * n, 2)))
+
+
+def from_arrays_periodic(arrays, order=ORDER):
+    """`pMPO` arrays (all 4-D) -> internal W[l, o, i, r]."""
+    return [np.ascontiguousarray(np.transpose(np.asarray(a, dtype=np.complex128),
+                                              [list(order).index(x) for x in ("l", "o", "i", "r")])) for a in arrays]
 
 
 def apply_dense(ws, v):
@@ -103,7 +110,7 @@ def apply_exact(ws, psi: oc.Chain) -> oc.Chain:
         t = np.einsum("woiv,bic->wbovc", w, a)
         D, B, d, V, Cc = t.shape
         out.append(t.reshape(D * B, d, V * Cc))
-    return oc.Chain(out)
+    return oc.Chain(out, periodic=psi.periodic)
 
 
 def _qr_cols(m):

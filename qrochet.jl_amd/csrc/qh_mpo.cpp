@@ -27,7 +27,8 @@
 struct qh_mpo {
     qh_context* ctx = nullptr;
     int n = 0, d = 0;
-    std::vector<int> D;               // n+1 entries, D[0] = D[n] = 1
+    bool periodic = false;            // `pMPO`: D[0] = D[n] = the wrap bond
+    std::vector<int> D;               // n+1 entries, D[0] = D[n] = 1 when open
     std::vector<qh::cplx*> w;         // W[w, o, i, v]
     std::vector<qh::cplx*> wr;        // (d·D_right) × (d·D_left): Wr[(i + d v), (o + d w)]
     std::vector<qh::cplx*> wl;        // (d·D_left) × (d·D_right): Wl[(i + d w), (o + d v)]
@@ -215,7 +216,8 @@ void mpo_fit_core(qh_context* c, const qh_mpo* h, const qh_mps* psi, qh_mps* phi
     check_mps(c, psi);
     check_mps(c, phi);
     if (psi == phi) throw Error(QH_ERR_ARGUMENT, "the fitted MPS must be distinct from the input MPS");
-    if (psi->periodic || phi->periodic) throw Error(QH_ERR_ARGUMENT, "MPO fitting is defined for open chains only");
+    if (psi->periodic || phi->periodic || h->periodic)
+        throw Error(QH_ERR_ARGUMENT, "MPO fitting is defined for open chains only");
     if (psi->n != h->n || phi->n != h->n) throw Error(QH_ERR_DIMENSION, "MPO and MPS lengths differ");
     if (psi->d != h->d || phi->d != h->d) throw Error(QH_ERR_DIMENSION, "physical dimensions differ");
     if (sweeps < 1) throw Error(QH_ERR_ARGUMENT, "sweeps must be positive");
@@ -298,9 +300,8 @@ void mpo_fit_core(qh_context* c, const qh_mpo* h, const qh_mps* psi, qh_mps* phi
 
 }  // namespace
 
-extern "C" {
-
-int qh_mpo_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out) {
+static int mpo_create_impl(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays,
+                           bool periodic, qh_mpo** out) {
     if (!c || !out) return QH_ERR_ARGUMENT;
     qh_mpo* h = nullptr;
     int rc = api_guard(c, [&] {
@@ -311,11 +312,13 @@ int qh_mpo_create(qh_context* c, int n, int d, const int* bond_dims, const doubl
         h->ctx = c;
         h->n = n;
         h->d = d;
+        h->periodic = periodic;
         h->D.assign(n + 1, 1);
-        for (int b = 1; b <= n - 1; ++b) {
+        for (int b = 1; b <= (periodic ? n : n - 1); ++b) {
             if (bond_dims[b - 1] < 1) throw Error(QH_ERR_DIMENSION, "bond dimensions must be positive");
             h->D[b] = bond_dims[b - 1];
         }
+        if (periodic) h->D[0] = h->D[n];
         h->w.assign(n, nullptr);
         h->wr.assign(n, nullptr);
         h->wl.assign(n, nullptr);
@@ -359,6 +362,16 @@ int qh_mpo_create(qh_context* c, int n, int d, const int* bond_dims, const doubl
     return QH_OK;
 }
 
+extern "C" {
+
+int qh_mpo_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out) {
+    return mpo_create_impl(c, n, d, bond_dims, arrays, false, out);
+}
+
+int qh_pmpo_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays, qh_mpo** out) {
+    return mpo_create_impl(c, n, d, bond_dims, arrays, true, out);
+}
+
 int qh_mpo_free(qh_mpo* h) {
     if (!h) return QH_OK;
     qh_context* c = h->ctx;
@@ -379,12 +392,14 @@ int qh_mpo_apply(qh_context* c, const qh_mpo* h, const qh_mps* psi, qh_mps** out
         check_mps(c, psi);
         if (psi->n != h->n) throw Error(QH_ERR_DIMENSION, "MPO and MPS lengths differ");
         if (psi->d != h->d) throw Error(QH_ERR_DIMENSION, "physical dimensions differ");
-        if (psi->periodic) throw Error(QH_ERR_ARGUMENT, "the MPO is open; the MPS is periodic");
+        if (psi->periodic != h->periodic)
+            throw Error(QH_ERR_ARGUMENT, "MPO and MPS must have the same boundary (both open or both periodic)");
         const int n = h->n, d = h->d;
         m = new qh_mps();
         m->ctx = c;
         m->n = n;
         m->d = d;
+        m->periodic = psi->periodic;
         m->chi.assign(n + 1, 1);
         for (int b = 0; b <= n; ++b) m->chi[b] = psi->chi[b] * h->D[b];
         m->site.assign(n, nullptr);

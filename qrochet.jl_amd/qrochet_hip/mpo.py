@@ -81,6 +81,33 @@ def MPO(arrays, order=_ORDER, context: Context | None = None) -> MPOChain:
     return MPOChain(h, ctx, n, d, dims)
 
 
+def pMPO(arrays, order=_ORDER, context: Context | None = None) -> MPOChain:
+    """`pMPO(arrays)` = `Chain(Operator(), Periodic(), arrays)` (src/Ansatz/Chain.jl:21,102-131):
+    every array 4-D; site n's right bond is site 1's left bond."""
+    ctx = context or default_context()
+    n = len(arrays)
+    if n < 2:
+        raise ArgumentError("an MPO needs at least 2 sites")
+    if sorted(order) != sorted(_ORDER):
+        raise ArgumentError("order must be a permutation of (o, i, l, r)")
+    if not all(np.ndim(a) == 4 for a in arrays):
+        raise AssertionError("All arrays must have 4 dimensions")
+    full = [np.transpose(np.asarray(a, dtype=np.complex128), [list(order).index(x) for x in _ORDER]) for a in arrays]
+    d = full[0].shape[0]
+    for k, f in enumerate(full):
+        if f.shape[0] != d or f.shape[1] != d:
+            raise DimensionMismatch("all sites must share the physical dimension")
+        if f.shape[3] != full[(k + 1) % n].shape[2]:
+            raise DimensionMismatch(f"bond {k + 1}: {f.shape[3]} != {full[(k + 1) % n].shape[2]}")
+    bufs = [np.asfortranarray(f).ravel(order="F").view(np.float64) for f in full]
+    dims = [f.shape[3] for f in full]
+    cdims = (C.c_int * n)(*dims)
+    ptrs = (L._dp * n)(*[b.ctypes.data_as(L._dp) for b in bufs])
+    h = C.c_void_p()
+    ctx.check(L.load().qh_pmpo_create(ctx.handle, n, d, cdims, ptrs, C.byref(h)))
+    return MPOChain(h, ctx, n, d, dims[:-1])
+
+
 def apply(mpo: MPOChain, psi: Chain) -> Chain:
     """H|ψ⟩ exactly, as a new MPS with bond dimensions χ_b·D_b."""
     out = C.c_void_p()

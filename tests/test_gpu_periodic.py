@@ -103,3 +103,19 @@ def test_pmps_open_only_methods(qh):
                  np.ones((2, 2)) / 2])
     with pytest.raises(qh.ArgumentError):
         qh.overlap(psi, op)
+
+
+def test_pmpo_apply_to_pmps(qh):
+    from oracle import mpo as om
+    from qrochet_hip import mpo as qm
+
+    rng = np.random.default_rng(45)
+    n, D = 5, 3
+    arrs = _random_pmps(rng, n, 2, 2)
+    marr = [(rng.standard_normal((2, 2, D, D)) + 1j * rng.standard_normal((2, 2, D, D))) for _ in range(n)]
+    out = qm.apply(qm.pMPO(marr), qh.pMPS(arrs))
+    assert out.periodic and out.wrap_dim() == 2 * D
+    exp = oc.statevector(om.apply_exact(om.from_arrays_periodic(marr), oc.pMPS(arrs)))
+    assert np.max(np.abs(_vec(out) - exp)) < 1e-10 * np.max(np.abs(exp))
+    with pytest.raises(qh.ArgumentError):
+        qm.apply(qm.pMPO(marr), qh.MPS([np.ones((2, 2))] + [np.ones((2, 2, 2))] * (n - 2) + [np.ones((2, 2))]))

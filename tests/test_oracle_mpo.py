@@ -109,3 +109,25 @@ def test_local_z_matches_expect():
     nrm = oc.norm(psi) ** 2
     ref = np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]) / nrm
     assert np.max(np.abs(z - ref)) < 1e-12
+
+
+def test_periodic_mpo_apply_matches_dense():
+    rng = np.random.default_rng(8)
+    n, w, D = 5, 2, 3
+    arrs = [(rng.standard_normal((2, w if i == 0 else 2, 2 if i < n - 1 else w))
+             + 1j * rng.standard_normal((2, w if i == 0 else 2, 2 if i < n - 1 else w))) for i in range(n)]
+    psi = oc.pMPS(arrs)
+    marr = [(rng.standard_normal((2, 2, D, D)) + 1j * rng.standard_normal((2, 2, D, D))) for _ in range(n)]
+    ws = om.from_arrays_periodic(marr)
+    hpsi = om.apply_exact(ws, psi)
+    assert hpsi.periodic
+    v = oc.statevector(psi)
+    exp = om.apply_dense(ws, v)
+    assert np.max(np.abs(oc.statevector(hpsi) - exp)) < 1e-12 * np.max(np.abs(exp))
+    # the dense periodic MPO of a product of single-site operators with D = 1 is their kron
+    ops = [wl.Z, wl.X, wl.Z, wl.I2, wl.X]
+    h = om.dense([op.reshape(1, 2, 2, 1) for op in ops]).reshape(32, 32)
+    k = ops[0]
+    for op in ops[1:]:
+        k = np.kron(k, op)
+    assert np.allclose(h, k)
