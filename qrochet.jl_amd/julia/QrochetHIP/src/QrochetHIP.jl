@@ -184,6 +184,59 @@ end
 
 simple_update!(h::HIPChain, gate::Dense; kwargs...) = Qrochet.evolve!(h, gate; iscanonical = true, kwargs...)
 
+"""
+    evolve_layer!(h::HIPChain, gates::Vector{<:Dense}; threshold, maxdim, iscanonical, renormalize, precision = :fp64)
+
+A layer of gates on disjoint lanes: one-site gates with `evolve!`, all two-site
+gates in one batched call (`qh_evolve2_batch`).  Same result as calling `evolve!`
+gate by gate.
+"""
+function evolve_layer!(h::HIPChain, gates::AbstractVector{<:Dense}; threshold = nothing, maxdim = nothing,
+                       iscanonical = false, renormalize = false, precision::Symbol = :fp64)
+    two = Dense[]
+    for g in gates
+        length(outputs(g)) == 1 ? Qrochet.evolve!(h, g; precision) : push!(two, g)
+    end
+    isempty(two) && return h
+    count = length(two)
+    bonds = Cint[]
+    buf = ComplexF64[]
+    for g in two
+        lanes = sort!(map(id, outputs(g)))
+        (length(lanes) == 2 && lanes[2] == lanes[1] + 1) || throw(ArgumentError("Gate lanes must be contiguous"))
+        push!(bonds, lanes[1])
+        append!(buf, vec(gate_buffer(g)))
+    end
+    flags = QH_PREC[precision] | (iscanonical ? QH_CANONICAL : UInt32(0)) | (renormalize ? QH_RENORMALIZE : UInt32(0))
+    thr = fill(threshold === nothing ? NaN : Float64(threshold), count)
+    md = fill(maxdim === nothing ? Cint(-1) : Cint(maxdim), count)
+    handles = fill(h.ptr, count)
+    check(h.ctx, ccall((:qh_evolve2_batch, libqh), Cint,
+                       (Ptr{Cvoid}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Cint}, Ptr{ComplexF64}, Ptr{Cdouble}, Ptr{Cint}, Cuint,
+                        Ptr{Cint}, Ptr{Cdouble}),
+                       h.ctx.ptr, count, handles, bonds, buf, thr, md, flags, C_NULL, C_NULL))
+    h
+end
+
+"""
+    layers(gates, lanes_of)
+
+Greedy as-soon-as-possible schedule of `gates` into layers of disjoint lanes that
+keeps the order of the gates on every lane (`lanes_of(g)` gives a gate's lanes).
+"""
+function layers(gates, lanes_of)
+    free = Dict{Int,Int}()
+    out = Vector{Vector{eltype(gates)}}()
+    for g in gates
+        ls = lanes_of(g)
+        k = maximum(l -> get(free, l, 1), ls)
+        k > length(out) && push!(out, eltype(gates)[])
+        push!(out[k], g)
+        foreach(l -> free[l] = k + 1, ls)
+    end
+    out
+end
+
 function Qrochet.truncate!(h::HIPChain, bond; threshold = nothing, maxdim = nothing)
     b1, b2 = id.(bond)
     b2 == b1 + 1 || throw(ArgumentError("Invalid bond $bond"))
@@ -253,6 +306,6 @@ for f in (:isleftcanonical, :isrightcanonical)
     end
 end
 
-export HIPChain, simple_update!
+export HIPChain, simple_update!, evolve_layer!
 
 end
