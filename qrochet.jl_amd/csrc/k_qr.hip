@@ -113,131 +113,26 @@ struct PanelShared {
     double2 sTau[QR_NB];
 };
 
-// panel factorisation streaming from global memory (any number of rows)
-__device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
-    double2* sRed = sh.sRed;
-    double2* sDot = sh.sDot;
-    auto& sT = sh.sT;
-    double2& sScale = sh.sScale;
-    double& sBeta = sh.sBeta;
-    double2* sTau = sh.sTau;
+// Column loop of a panel factorisation on a panel view P(c, r) (column c, row r
+// relative to the panel corner), shared by the LDS-resident and the in-place
+// global-memory variants.  Leaves R on and above the diagonal, the scaled
+// reflectors below it, and the compact-WY triangle in sh.sT.
+template <class Ref>
+__device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
     const int tid = threadIdx.x;
-    const int m = it.m;
-    const int nb = min(QR_NB, it.n - j0);
-    double2* a = it.a;
-    double2* v = it.v;
-    for (int c = 0; c < nb; ++c) {
-        const int col = j0 + c, row0 = j0 + c;
-        // ---- reflector for a[row0:, col]
-        double2 part[1] = {cz()};
-        for (int r = row0 + 1 + tid; r < m; r += 256) part[0].x += cabs2(a[r + (size_t)it.lda * col]);
-        block_sum<1>(part, sRed, sDot);
-        if (tid == 0) {
-            const double xnorm2 = sDot[0].x;
-            const double2 alpha = a[row0 + (size_t)it.lda * col];
-            double2 tau = cz(), scale = cz();
-            double beta = alpha.x;
-            if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
-                const double nrm = sqrt(cabs2(alpha) + xnorm2);
-                beta = alpha.x >= 0.0 ? -nrm : nrm;
-                tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-                // scale = 1 / (alpha - beta)
-                const double2 den = make_double2(alpha.x - beta, alpha.y);
-                const double dd = cabs2(den);
-                scale = make_double2(den.x / dd, -den.y / dd);
-            }
-            sTau[c] = tau;
-            sScale = scale;
-            sBeta = beta;
-            a[row0 + (size_t)it.lda * col] = make_double2(beta, 0.0);
-        }
-        __syncthreads();
-        const double2 scale = sScale;
-        for (int r = j0 + tid; r < m; r += 256) {
-            double2 vr;
-            if (r < row0) vr = cz();
-            else if (r == row0) vr = make_double2(1.0, 0.0);
-            else {
-                vr = cmul(a[r + (size_t)it.lda * col], scale);
-                a[r + (size_t)it.lda * col] = cz();
-            }
-            v[r + (size_t)it.m * col] = vr;
-        }
-        __syncthreads();
-        // ---- dots: w_k = v^H a_k (k > c), s_i = v_i^H v (i < c)
-        double2 dots[QR_NB];
-#pragma unroll
-        for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
-        for (int r = row0 + tid; r < m; r += 256) {
-            const double2 vr = v[r + (size_t)it.m * col];
-#pragma unroll
-            for (int k = 0; k < QR_NB; ++k) {
-                if (k > c && k < nb) dots[k] = cfmac(vr, a[r + (size_t)it.lda * (j0 + k)], dots[k]);
-                else if (k < c) dots[k] = cfmac(v[r + (size_t)it.m * (j0 + k)], vr, dots[k]);
-            }
-        }
-        block_sum<QR_NB>(dots, sRed, sDot);
-        // ---- T column c:  T[i][c] = -tau_c Σ_{l=i}^{c-1} T[i][l] s_l ;  T[c][c] = tau_c
-        if (tid < QR_NB) {
-            const int i = tid;
-            double2 val = cz();
-            if (i < c) {
-                double2 acc = cz();
-                for (int l = i; l < c; ++l) acc = cfma(sT[i][l], sDot[l], acc);
-                const double2 tc = sTau[c];
-                val = make_double2(-(tc.x * acc.x - tc.y * acc.y), -(tc.x * acc.y + tc.y * acc.x));
-            } else if (i == c) {
-                val = sTau[c];
-            }
-            sT[i][c] = val;
-        }
-        // ---- update remaining panel columns: a_k -= conj(tau) w_k v
-        const double2 ctau = cconj(sTau[c]);
-        for (int r = row0 + tid; r < m; r += 256) {
-            const double2 vr = v[r + (size_t)it.m * col];
-            for (int k = c + 1; k < nb; ++k) {
-                const double2 f = cmul(ctau, sDot[k]);
-                double2& y = a[r + (size_t)it.lda * (j0 + k)];
-                y = csub(y, cmul(f, vr));
-            }
-        }
-        __syncthreads();
-    }
-    // store T (nb × nb, column-major, ld QR_NB)
-    double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
-    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
-        const int i = e % QR_NB, j = e / QR_NB;
-        t[i + QR_NB * j] = (i < nb && j < nb) ? sT[i][j] : cz();
-    }
-}
-
-
-constexpr int QR_LDS_ROWS = 512;  // panels of up to this many rows are factored in LDS
-
-// panel factorisation with the whole panel resident in LDS (rows j0..m-1 <= QR_LDS_ROWS)
-__device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 (*sP)[QR_LDS_ROWS]) {
-    const int tid = threadIdx.x;
-    const int m = it.m;
-    const int rows = m - j0;
-    const int nb = min(QR_NB, it.n - j0);
-    for (int e = tid; e < QR_NB * rows; e += 256) {
-        const int c = e / rows, r = e % rows;
-        sP[c][r] = (c < nb) ? it.a[(j0 + r) + (size_t)it.lda * (j0 + c)] : cz();
-    }
-    __syncthreads();
     // Thread tid owns rows r = tid + 256 j for the whole panel, so the reflector's
     // scaling, the dots, the trailing update and the next column's norm touch only
     // the thread's own rows; the only cross-thread data are the reductions and
-    // alpha = sP[c][c], whose owner writes it before the barrier of the next norm.
+    // alpha = P(c, c), whose owner writes it before the barrier of the next norm.
     double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
     for (int r = tid; r < rows; r += 256)
-        if (r > 0) part += cabs2(sP[0][r]);
+        if (r > 0) part += cabs2(P(0, r));
     for (int c = 0; c < nb; ++c) {
         // every thread forms the reflector from the reduced norm (no serial section);
         // the two norm buffers alternate so a fast thread cannot overwrite partial
         // sums that a slow thread of the previous column still reads
         const double xnorm2 = block_sum_all(part, sh.sRedN[c & 1]);
-        const Reflector h = make_reflector(sP[c][c], xnorm2);
+        const Reflector h = make_reflector(P(c, c), xnorm2);
         // v below the diagonal (scaled in place) and its dots with the panel:
         // dots[k] = v^H a_k (k > c), dots[k] = v_k^H v (k < c)
         double2 dots[QR_NB];
@@ -247,21 +142,21 @@ __device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 
             if (r < c) continue;
             double2 vr = make_double2(1.0, 0.0);
             if (r > c) {
-                vr = cmul(sP[c][r], h.scale);
-                sP[c][r] = vr;
+                vr = cmul(P(c, r), h.scale);
+                P(c, r) = vr;
             }
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
-                if (k > c && k < nb) dots[k] = cfmac(vr, sP[k][r], dots[k]);
+                if (k > c && k < nb) dots[k] = cfmac(vr, P(k, r), dots[k]);
                 else if (k < c) {
-                    const double2 vk = sP[k][r];  // r >= c > k: below column k's diagonal
+                    const double2 vk = P(k, r);  // r >= c > k: below column k's diagonal
                     dots[k] = cfmac(vk, vr, dots[k]);
                 }
             }
         }
         block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
-        // (every thread has read alpha = sP[c][c] before the barriers above)
-        if (tid == c) sP[c][c] = make_double2(h.beta, 0.0);
+        // (every thread has read alpha = P(c, c) before the barriers above)
+        if (tid == c) P(c, c) = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
             const int i = tid;
             double2 val = cz();
@@ -284,18 +179,65 @@ __device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 
         part = 0.0;
         for (int r = tid; r < rows; r += 256) {
             if (r < c) continue;
-            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : sP[c][r];
+            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : P(c, r);
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
                 if (k > c && k < nb) {
-                    const double2 x = csub(sP[k][r], cmul(w[k], vr));
-                    sP[k][r] = x;
+                    const double2 x = csub(P(k, r), cmul(w[k], vr));
+                    P(k, r) = x;
                     if (k == c + 1 && r > c + 1) part += cabs2(x);
                 }
             }
         }
     }
     __syncthreads();
+}
+
+// panel factorisation in place in global memory (any number of rows; the panel's
+// 16 columns stay in L2 across the column loop)
+__device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
+    const int tid = threadIdx.x;
+    const int rows = it.m - j0;
+    const int nb = min(QR_NB, it.n - j0);
+    double2* base = it.a + j0 + (size_t)it.lda * j0;
+    const size_t lda = it.lda;
+    qr_panel_columns(rows, nb, sh, [&](int c, int r) -> double2& { return base[r + lda * c]; });
+    __syncthreads();
+    // explicit reflectors to v, zeros below the diagonal of R
+    for (int e = tid; e < nb * rows; e += 256) {
+        const int c = e / rows, r = e % rows;
+        const size_t vi = (j0 + r) + (size_t)it.m * (j0 + c);
+        double2& ar = base[r + lda * c];
+        if (r < c) {
+            it.v[vi] = cz();
+        } else if (r == c) {
+            it.v[vi] = make_double2(1.0, 0.0);
+        } else {
+            it.v[vi] = ar;
+            ar = cz();
+        }
+    }
+    double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
+    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+        const int i = e % QR_NB, j = e / QR_NB;
+        t[i + QR_NB * j] = (i < nb && j < nb) ? sh.sT[i][j] : cz();
+    }
+}
+
+constexpr int QR_LDS_ROWS = 512;  // panels of up to this many rows are factored in LDS
+
+// panel factorisation with the whole panel resident in LDS (rows j0..m-1 <= QR_LDS_ROWS)
+__device__ void qr_panel_lds(const QrItem& it, int j0, PanelShared& sh, double2 (*sP)[QR_LDS_ROWS]) {
+    const int tid = threadIdx.x;
+    const int m = it.m;
+    const int rows = m - j0;
+    const int nb = min(QR_NB, it.n - j0);
+    for (int e = tid; e < QR_NB * rows; e += 256) {
+        const int c = e / rows, r = e % rows;
+        sP[c][r] = (c < nb) ? it.a[(j0 + r) + (size_t)it.lda * (j0 + c)] : cz();
+    }
+    __syncthreads();
+    qr_panel_columns(rows, nb, sh, [&](int c, int r) -> double2& { return sP[c][r]; });
     // write R (upper part) back to a, the explicit reflectors to v
     for (int e = tid; e < nb * rows; e += 256) {
         const int c = e / rows, r = e % rows;

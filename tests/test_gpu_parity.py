@@ -376,3 +376,25 @@ def test_batched_layer_with_multichunk_descriptor_arena(qh, monkeypatch):
         assert a.bond_dims() == b.bond_dims()
         for x, y in zip(a.internal_arrays(), b.internal_arrays()):
             assert np.array_equal(x, y)
+
+
+def test_theta_taller_than_lds_panel(qh):
+    """A 640×640 θ (χ = 320): the QR preconditioner factors its panels in place in
+    global memory (panels of more than 512 rows do not fit the LDS variant)."""
+    from oracle import mpo as om
+
+    rng = np.random.default_rng(23)
+    n = 20
+    arrays = _random_chain(rng, n, 2, 320)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    g = wl.haar_gate4(rng)
+    info = {}
+    qh.evolve_(psi, qh.gate(g, 10, 11), maxdim=300, info=info)
+    oc.evolve(ref, oc.gate2(g, 10), maxdim=300)
+    assert psi.bond_dims() == ref.bond_dims()
+    lg, lr = psi.lambda_(10), ref.lam[9]
+    assert np.max(np.abs(lg - lr)) < 1e-10 * lr[0]
+    z = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in (9, 10, 11, 12)]) / qh.norm(psi) ** 2
+    zr = om.local_z(ref)[[8, 9, 10, 11]]
+    assert np.max(np.abs(z - zr)) < 1e-10
