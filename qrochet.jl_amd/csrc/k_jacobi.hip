@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
-    constexpr int TS = BC / 8;   // Gram sub-tile per thread (8 × 8 threads per wave)
+    constexpr int S = B / 8;     // Gram sub-tile per thread and block (8 × 8 threads per wave)
     constexpr int NP = BC / 2;   // rotations per inner round
     __shared__ double2 sG[BC][LP];
     __shared__ double2 sQ[BC][LP];
@@ -108,11 +108,13 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     // ---------------- Gram matrix ----------------
     const int lane = tid & 63, wv = tid >> 6;
     const int ti = lane & 7, tj = lane >> 3;
-    double2 acc[TS][TS];
+    // G = [[G11, G12], [G12^H, G22]] over the two column blocks: only G11, G12 and
+    // G22 are accumulated (3/4 of the products); G21 comes from G12 below
+    double2 acc11[S][S], acc12[S][S], acc22[S][S];
 #pragma unroll
-    for (int a = 0; a < TS; ++a)
+    for (int a = 0; a < S; ++a)
 #pragma unroll
-        for (int b = 0; b < TS; ++b) acc[a][b] = cz();
+        for (int b = 0; b < S; ++b) acc11[a][b] = acc12[a][b] = acc22[a][b] = cz();
 
     // chunk loads are software-pipelined: chunk k+1 is fetched into registers
     // while chunk k is reduced from LDS
@@ -141,27 +143,37 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
         if (row0 + CH < p) fetch(it.x, p, row0 + CH);
 #pragma unroll 2
         for (int rr = wv; rr < CH; rr += 4) {
-            double2 xi[TS], xj[TS];
+            double2 x1i[S], x1j[S], x2i[S], x2j[S];
 #pragma unroll
-            for (int a = 0; a < TS; ++a) {
-                xi[a] = sX[ti * TS + a][rr];
-                xj[a] = sX[tj * TS + a][rr];
+            for (int a = 0; a < S; ++a) {
+                x1i[a] = sX[ti * S + a][rr];
+                x1j[a] = sX[tj * S + a][rr];
+                x2i[a] = sX[B + ti * S + a][rr];
+                x2j[a] = sX[B + tj * S + a][rr];
             }
 #pragma unroll
-            for (int a = 0; a < TS; ++a)
+            for (int a = 0; a < S; ++a)
 #pragma unroll
-                for (int b = 0; b < TS; ++b) acc[a][b] = cfmac(xi[a], xj[b], acc[a][b]);
+                for (int b = 0; b < S; ++b) {
+                    acc11[a][b] = cfmac(x1i[a], x1j[b], acc11[a][b]);
+                    acc12[a][b] = cfmac(x1i[a], x2j[b], acc12[a][b]);
+                    acc22[a][b] = cfmac(x2i[a], x2j[b], acc22[a][b]);
+                }
         }
         __syncthreads();
     }
     for (int w = 0; w < 4; ++w) {
         if (wv == w) {
 #pragma unroll
-            for (int a = 0; a < TS; ++a)
+            for (int a = 0; a < S; ++a)
 #pragma unroll
-                for (int b = 0; b < TS; ++b) {
-                    double2& g = sG[ti * TS + a][tj * TS + b];
-                    g = (w == 0) ? acc[a][b] : cadd(g, acc[a][b]);
+                for (int b = 0; b < S; ++b) {
+                    double2& g11 = sG[ti * S + a][tj * S + b];
+                    double2& g12 = sG[ti * S + a][B + tj * S + b];
+                    double2& g22 = sG[B + ti * S + a][B + tj * S + b];
+                    g11 = (w == 0) ? acc11[a][b] : cadd(g11, acc11[a][b]);
+                    g12 = (w == 0) ? acc12[a][b] : cadd(g12, acc12[a][b]);
+                    g22 = (w == 0) ? acc22[a][b] : cadd(g22, acc22[a][b]);
                 }
         }
         __syncthreads();
