@@ -70,8 +70,17 @@ def serving_leg(kind, requests, qps, seed=2025):
     reqs = sv.make_requests(seed, requests, qps, **dist)
     srv = sv.Server()
     t0 = time.perf_counter()
+    last = [0.0]
+
+    def on_layer(lane, running, updates):  # at most one line per 20 s
+        now = time.perf_counter() - t0
+        if now - last[0] > 20.0:
+            last[0] = now
+            log(f"t={now:.0f} s: lane {lane} finished a layer ({running} requests running)")
+
     res = srv.run(reqs, realtime=True,
-                  log=lambda r: log(f"request {r.rid} {r.status} latency {r.latency:.2f} s ({r.updates} updates)"))
+                  log=lambda r: log(f"request {r.rid} {r.status} latency {r.latency:.2f} s ({r.updates} updates)"),
+                  log_iter=on_layer)
     wall = time.perf_counter() - t0
     ok = [r for r in res if r.status == "ok"]
     updates = sum(r.updates for r in ok)

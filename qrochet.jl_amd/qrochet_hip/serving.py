@@ -240,6 +240,8 @@ class _Lane(threading.Thread):
                 continue
             self._step()
             srv._count("iterations")
+            if srv._log_iter:
+                srv._log_iter(self.index, len(self.active), sum(a.res.updates for a in self.active))
             done = [a for a in self.active if a.layer >= a.req.depth or a.res.status != "ok"]
             for a in done:
                 if a.res.status == "ok":
@@ -322,10 +324,12 @@ class Server:
         if self._log:
             self._log(res)
 
-    def run(self, requests, realtime: bool = True, log=None) -> list[Result]:
+    def run(self, requests, realtime: bool = True, log=None, log_iter=None) -> list[Result]:
         """Serve ``requests`` (arrival times in seconds after the call).  With
         ``realtime`` False every request is taken as arrived at once, in arrival
-        order (values are the same; latencies are then not meaningful)."""
+        order (values are the same; latencies are then not meaningful).  ``log(result)``
+        is called per finished request, ``log_iter(lane, running, updates)`` after
+        every batched layer of a lane."""
         engines = [self.engine_factory() for _ in self.lane_bounds]
         if self.budget is None:
             self.budget = int(self._hbm_budget if self._hbm_budget is not None
@@ -336,6 +340,7 @@ class Server:
         self._used = 0
         self._inflight = 0
         self._log = log
+        self._log_iter = log_iter
         self.closing = False
         t0 = time.perf_counter()
         self._now = lambda: time.perf_counter() - t0
