@@ -158,6 +158,61 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
 // ---------------------------------------------------------------------------
 // factorisations of one site (Chain.jl:339-376)
 // ---------------------------------------------------------------------------
+// `canonize_site!(…; method = :qr)` (Chain.jl:339-376) with a Householder QR like
+// LAPACK's `qr!`: RIGHT: A (χl d × χr) = Q R, Q -> site, R Λ A_{i+1} -> site+1;
+// LEFT: A (χl × d χr) = R^H Q^H from the QR of A^H, Q^H -> site, A_{i-1} Λ R^H -> site-1.
+// The bond dimension stays χr (RIGHT) / χl (LEFT); the bond's Λ is absorbed.
+void householder_site(qh_context* c, qh_mps* m, int site, int direction, int mr, int mc) {
+    const int d = m->d;
+    const int rows = direction == QH_RIGHT ? mr : mc;  // the tall matrix factored
+    const int cols = direction == QH_RIGHT ? mc : mr;
+    const cplx* a = m->site[site - 1];
+    cplx* x = nullptr;
+    if (direction == QH_RIGHT) x = dcopy(c, a, (size_t)mr * mc);
+    else {
+        x = dalloc<cplx>(c, (size_t)mr * mc);
+        launch_conj_transpose(a, mr, mc, x, c->stream);  // A^H: (d χr) × χl
+    }
+    const size_t tq = (size_t)((cols + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
+    cplx* v = dalloc<cplx>(c, (size_t)rows * cols);
+    cplx* t = dalloc<cplx>(c, tq);
+    std::vector<QrJob> jobs{QrJob{x, rows, cols, rows, v, t}};
+    run_qr_factor(c, jobs);  // R in the upper triangle of x (ld rows), zeros below
+    cplx* q = dalloc<cplx>(c, (size_t)rows * cols);
+    launch_eye_rect(q, rows, cols, c->stream);
+    std::vector<QrApplyJob> app{QrApplyJob{&jobs[0], q, cols}};
+    run_qr_apply_q(c, app);
+    if (direction == QH_RIGHT) {
+        const int nb = site + 1;
+        const int n2 = d * m->chi[nb];
+        cplx* nn = dalloc<cplx>(c, (size_t)cols * n2);
+        std::vector<GemmItem> g{gemm_item(x, GEMM_N, rows, m->site[nb - 1], GEMM_N, cols, nn, cols, cols, n2, cols)};
+        g[0].dk = m->lam[site - 1];
+        g[0].dk_mod = cols;
+        run_gemm(c, g);
+        replace_site(m, site, q);
+        replace_site(m, nb, nn);
+        replace_lam(m, site, nullptr);
+    } else {
+        const int pb = site - 1;
+        const int m2 = m->chi[pb - 1] * d;
+        cplx* na = dalloc<cplx>(c, (size_t)cols * rows);
+        launch_conj_transpose(q, rows, cols, na, c->stream);  // Q^H: χl × (d χr)
+        cplx* nn = dalloc<cplx>(c, (size_t)m2 * cols);
+        std::vector<GemmItem> g{gemm_item(m->site[pb - 1], GEMM_N, m2, x, GEMM_C, rows, nn, m2, m2, cols, cols)};
+        g[0].dk = m->lam[site - 2];
+        g[0].dk_mod = cols;
+        run_gemm(c, g);
+        dfree(c, q);
+        replace_site(m, site, na);
+        replace_site(m, pb, nn);
+        replace_lam(m, pb, nullptr);
+    }
+    dfree(c, x);
+    dfree(c, v);
+    dfree(c, t);
+}
+
 void canonize_site_core(qh_context* c, qh_mps* m, int site, int direction, int method) {
     check_open(m, "canonize_site!");
     check_site(m, site);
@@ -176,6 +231,10 @@ void canonize_site_core(qh_context* c, qh_mps* m, int site, int direction, int m
     const int mc = direction == QH_RIGHT ? chir : d * chir;
     const int k = std::min(mr, mc);
     const cplx* a = m->site[site - 1];
+    if (method == QH_METHOD_QR && ((direction == QH_RIGHT && mr >= mc) || (direction == QH_LEFT && mc >= mr))) {
+        householder_site(c, m, site, direction, mr, mc);
+        return;
+    }
     cplx* x = nullptr;
     if (mr >= mc) x = dcopy(c, a, (size_t)mr * mc);
     else {

@@ -398,3 +398,30 @@ def test_theta_taller_than_lds_panel(qh):
     z = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in (9, 10, 11, 12)]) / qh.norm(psi) ** 2
     zr = om.local_z(ref)[[8, 9, 10, 11]]
     assert np.max(np.abs(z - zr)) < 1e-10
+
+
+def test_workload2_full_size_time_reversal(qh):
+    """Workload 2 at its full size (n = 64, chi <= 256, cutoff 1e-12): 10 Strang steps
+    forward, then the same layers in reverse order with the inverse gates.  The
+    round trip returns to the initial product state up to the truncation: the
+    infidelity is bounded by the summed Eckart-Young errors of all cuts (each cut
+    moves the state by at most its discarded norm; the gates are unitary)."""
+    arrays, layers, trunc, _ = wl.config2(steps=10)
+    psi = qh.from_internal(arrays)
+    psi0 = psi.copy()
+    disc = 0.0
+    for layer in layers:
+        _, d = qh.evolve_layer_([(psi, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
+                                maxdim=trunc["maxdim"])
+        disc += sum(d)
+    assert max(psi.bond_dims()) > 64  # the state became entangled
+    for layer in reversed(layers):
+        inv = [(b, g.reshape(4, 4, order="F").conj().T.reshape((2, 2, 2, 2), order="F")) for b, g in layer]
+        _, d = qh.evolve_layer_([(psi, qh.gate(g, b, b + 1)) for b, g in inv], threshold=trunc["threshold"],
+                                maxdim=trunc["maxdim"])
+        disc += sum(d)
+    fid = abs(qh.overlap(psi, psi0))
+    print(f"return fidelity {fid:.15f}, summed discarded norm {disc:.3e}")
+    assert fid <= 1.0 + 1e-10
+    assert 1.0 - fid <= disc + 1e-10
+    assert 1.0 - fid < 1e-5
