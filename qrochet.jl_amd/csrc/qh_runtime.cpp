@@ -92,6 +92,7 @@ void* pool_alloc(qh_context* c, size_t bytes) {
     auto& fl3 = P.free[cs];
     void* p = fl3.back();
     fl3.pop_back();
+    P.live.insert(p);
     return p;
 }
 
@@ -108,6 +109,9 @@ void* dalloc_bytes(qh_context* c, size_t bytes) {
 void dfree(qh_context* c, void* p) {
     if (!p) return;
     c->call_allocs.erase(p);
+    // only a block that is handed out returns to its free list (a second free of
+    // the same pointer is ignored instead of handing the block out twice)
+    if (!c->pool.live.erase(p)) return;
     auto it = c->pool.size_of.find(p);
     if (it != c->pool.size_of.end()) c->pool.free[it->second].push_back(p);
 }
@@ -141,6 +145,7 @@ void pool_destroy(qh_context* c) {
     P.slabs.clear();
     P.free.clear();
     P.size_of.clear();
+    P.live.clear();
     P.reserved = 0;
 }
 

@@ -81,6 +81,7 @@ struct qh_context {
     struct Pool {
         std::unordered_map<size_t, std::vector<void*>> free;  // class size -> free blocks
         std::unordered_map<void*, size_t> size_of;            // block -> class size
+        std::unordered_set<void*> live;                       // blocks handed out
         std::vector<void*> slabs;                             // carved slabs (small classes)
         std::vector<void*> big;                               // blocks allocated one by one
         size_t reserved = 0;                                  // bytes obtained from the runtime
