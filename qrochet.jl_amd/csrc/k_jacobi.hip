@@ -76,9 +76,14 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     constexpr int CH = 32;
     constexpr int S = B / 8;     // Gram sub-tile per thread and block (8 × 8 threads per wave)
     constexpr int NP = BC / 2;   // rotations per inner round
+    // the update stages 128-row × 8-column panels of X through the same LDS as the
+    // Gram's 32-row chunks
+    constexpr int RB = 128, IC = 8;
+    constexpr int SX_ELEMS = (BC * (CH + 1) > IC * (RB + 1)) ? BC * (CH + 1) : IC * (RB + 1);
     __shared__ double2 sG[BC][LP];
     __shared__ double2 sQ[BC][LP];
-    __shared__ double2 sX[BC][CH + 1];
+    __shared__ double2 sXraw[SX_ELEMS];
+    double2 (*sX)[CH + 1] = reinterpret_cast<double2 (*)[CH + 1]>(sXraw);
     __shared__ int sCol[BC];
     __shared__ int sPerm[BC];
     __shared__ int sFlag;
@@ -322,33 +327,72 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     }
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
-    constexpr int JG = BC / 8;  // output columns per thread
-    const int rr = tid & 31, jg = tid >> 5;
+    // Register-blocked: thread (rg, cg) accumulates rows rg + 32a (a < RA) × columns
+    // cg·JG + b (b < JG) of a 128-row panel, so every LDS value feeds RA or JG complex
+    // FMAs.  The panel's X is staged IC inner columns at a time (prefetched into
+    // registers one stage ahead); the sum runs over i = 0..BC-1 in order.
+    constexpr int JG = BC / 8;      // output columns per thread
+    constexpr int RA = RB / 32;     // output rows per thread
+    constexpr int UPT = IC * RB / 256;
+    double2 (*sU)[RB + 1] = reinterpret_cast<double2 (*)[RB + 1]>(sXraw);
+    const int rg = tid & 31, cg = tid >> 5;
+    double2 upre[UPT];
+    auto ufetch = [&](const double2* base, int rows, int row0, int i0) {
+#pragma unroll
+        for (int s2 = 0; s2 < UPT; ++s2) {
+            const int e = tid + 256 * s2;
+            const int gc = sCol[i0 + e / RB], grow = row0 + e % RB;
+            upre[s2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
+        }
+    };
+    auto ustash = [&]() {
+#pragma unroll
+        for (int s2 = 0; s2 < UPT; ++s2) {
+            const int e = tid + 256 * s2;
+            sU[e / RB][e % RB] = upre[s2];
+        }
+    };
 #pragma unroll 1
     for (int mat = 0; mat < 2; ++mat) {
         double2* base = mat ? it.v : it.x;
         const int rows = mat ? it.q : p;
-        fetch(base, rows, 0);
-        for (int row0 = 0; row0 < rows; row0 += CH) {
-            stash();
-            __syncthreads();
-            if (row0 + CH < rows) fetch(base, rows, row0 + CH);
-            double2 o[JG];
+        ufetch(base, rows, 0, 0);
+#pragma unroll 1
+        for (int row0 = 0; row0 < rows; row0 += RB) {
+            double2 o[RA][JG];
 #pragma unroll
-            for (int a = 0; a < JG; ++a) o[a] = cz();
-#pragma unroll 4
-            for (int i = 0; i < BC; ++i) {
-                const double2 xv = sX[i][rr];
+            for (int a = 0; a < RA; ++a)
 #pragma unroll
-                for (int a = 0; a < JG; ++a) o[a] = cfma(xv, sQ[i][jg * JG + a], o[a]);
+                for (int b = 0; b < JG; ++b) o[a][b] = cz();
+#pragma unroll 1
+            for (int i0 = 0; i0 < BC; i0 += IC) {
+                ustash();
+                __syncthreads();
+                if (i0 + IC < BC) ufetch(base, rows, row0, i0 + IC);
+                else if (row0 + RB < rows) ufetch(base, rows, row0 + RB, 0);
+#pragma unroll 2
+                for (int i = 0; i < IC; ++i) {
+                    double2 xv[RA], qv[JG];
+#pragma unroll
+                    for (int a = 0; a < RA; ++a) xv[a] = sU[i][rg + 32 * a];
+#pragma unroll
+                    for (int b = 0; b < JG; ++b) qv[b] = sQ[i0 + i][cg * JG + b];
+#pragma unroll
+                    for (int a = 0; a < RA; ++a)
+#pragma unroll
+                        for (int b = 0; b < JG; ++b) o[a][b] = cfma(xv[a], qv[b], o[a][b]);
+                }
+                __syncthreads();
             }
-            const int grow = row0 + rr;
 #pragma unroll
-            for (int a = 0; a < JG; ++a) {
-                const int gc = sCol[jg * JG + a];
-                if (gc >= 0 && grow < rows) base[grow + (size_t)rows * gc] = o[a];
+            for (int a = 0; a < RA; ++a) {
+                const int grow = row0 + rg + 32 * a;
+#pragma unroll
+                for (int b = 0; b < JG; ++b) {
+                    const int gc = sCol[cg * JG + b];
+                    if (gc >= 0 && grow < rows) base[grow + (size_t)rows * gc] = o[a][b];
+                }
             }
-            __syncthreads();
         }
     }
 }

@@ -777,6 +777,7 @@ int qh_ctx_create(int device, qh_context** out) {
             const long long v = std::atoll(e);
             if (v >= 256) c->arena_first = (size_t)v;
         }
+        if (const char* e = std::getenv("QH_JACOBI_PRECONDITION")) c->jac_precondition = std::atoi(e);
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t thr = UINT64_MAX;

@@ -3,6 +3,7 @@
 #include "qh_runtime.h"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -554,6 +555,7 @@ int SvdTask::kmax() const {
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     if (tasks.empty()) return;
     const size_t nt = tasks.size();
+    const int npass = std::min(std::max(c->jac_precondition, 0), QR_MAX_PASSES);
     // workspace layout per task
     size_t bytes = 0;
     auto take = [&](size_t b) {
@@ -562,7 +564,8 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         return o;
     };
     struct Off {
-        size_t v, s, p, l, v1, t1, y0, v2, t2, yj, ux, vx;
+        size_t v, s, p, l, ux, vx, tmp;
+        size_t qv[QR_MAX_PASSES + 1], qt[QR_MAX_PASSES + 1], y[QR_MAX_PASSES + 1];
         bool pre;
         int p_, q_, k_;
     };
@@ -573,22 +576,22 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         o.p_ = std::max(t.mr, t.mc);
         o.q_ = std::min(t.mr, t.mc);
         o.k_ = t.kmax();
-        const size_t p = o.p_, q = o.q_, k = o.k_;
-        o.pre = c->jac_precondition && (int)q > 2 * c->jac_block;
+        const size_t p = o.p_, q = o.q_, k = std::max(o.k_, 1);
+        o.pre = npass > 0 && (int)q > 2 * c->jac_block;
         o.v = take(q * q * sizeof(cplx));
         o.s = take(q * sizeof(double));
         o.p = take(q * sizeof(int));
         o.l = take(q * sizeof(double));
         if (o.pre) {
             const size_t tq = ((q + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
-            o.v1 = take(p * q * sizeof(cplx));
-            o.t1 = take(tq * sizeof(cplx));
-            o.y0 = take(q * q * sizeof(cplx));
-            o.v2 = take(q * q * sizeof(cplx));
-            o.t2 = take(tq * sizeof(cplx));
-            o.yj = take(q * q * sizeof(cplx));
-            o.ux = take(p * std::max<size_t>(k, 1) * sizeof(cplx));
-            o.vx = take(q * std::max<size_t>(k, 1) * sizeof(cplx));
+            for (int j = 1; j <= npass; ++j) {
+                o.qv[j] = take((j == 1 ? p : q) * q * sizeof(cplx));
+                o.qt[j] = take(tq * sizeof(cplx));
+                o.y[j] = take(q * q * sizeof(cplx));
+            }
+            o.ux = take(p * k * sizeof(cplx));
+            o.vx = take(q * k * sizeof(cplx));
+            if (npass >= 3) o.tmp = take(q * k * sizeof(cplx));
         }
     }
     const size_t off_k = take(nt * sizeof(int));
@@ -598,24 +601,23 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     int* d_kept = reinterpret_cast<int*>(ws + off_k);
     double* d_disc = reinterpret_cast<double*>(ws + off_d);
 
-    // ---- QR preconditioning: X = Q1 R1, R1^H = Q2 R2, Jacobi on Y = R2^H
-    std::vector<QrJob> qr1(nt), qr2(nt);
-    std::vector<QrJob> f1, f2;
-    for (size_t i = 0; i < nt; ++i) {
-        if (!off[i].pre) continue;
-        const int p = off[i].p_, q = off[i].q_;
-        qr1[i] = QrJob{tasks[i].x, p, q, p, P(off[i].v1), P(off[i].t1)};
-        qr2[i] = QrJob{P(off[i].y0), q, q, q, P(off[i].v2), P(off[i].t2)};
-        f1.push_back(qr1[i]);
-        f2.push_back(qr2[i]);
-    }
-    if (!f1.empty()) {
-        run_qr_factor(c, f1);
+    // ---- QR preconditioning: X = Q1 R1 and R_{j-1}^H = Q_j R_j for j = 2..npass;
+    // the Jacobi runs on Y = R_npass^H (lower triangular), y[j] = R_j^H
+    std::vector<std::array<QrJob, QR_MAX_PASSES + 1>> qr(nt);
+    for (int j = 1; j <= npass; ++j) {
+        std::vector<QrJob> f;
+        for (size_t i = 0; i < nt; ++i) {
+            const Off& o = off[i];
+            if (!o.pre) continue;
+            qr[i][j] = (j == 1) ? QrJob{tasks[i].x, o.p_, o.q_, o.p_, P(o.qv[1]), P(o.qt[1])}
+                                : QrJob{P(o.y[j - 1]), o.q_, o.q_, o.q_, P(o.qv[j]), P(o.qt[j])};
+            f.push_back(qr[i][j]);
+        }
+        if (f.empty()) break;
+        run_qr_factor(c, f);
         for (size_t i = 0; i < nt; ++i)
-            if (off[i].pre) launch_upper_h(tasks[i].x, off[i].p_, off[i].q_, P(off[i].y0), c->stream);
-        run_qr_factor(c, f2);
-        for (size_t i = 0; i < nt; ++i)
-            if (off[i].pre) launch_upper_h(P(off[i].y0), off[i].q_, off[i].q_, P(off[i].yj), c->stream);
+            if (off[i].pre)
+                launch_upper_h(qr[i][j].a, qr[i][j].m, off[i].q_, P(off[i].y[j]), c->stream);
     }
 
     std::vector<JacItem> jac(nt);
@@ -624,7 +626,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         const SvdTask& t = tasks[i];
         const Off& o = off[i];
         jac[i] = JacItem{};
-        jac[i].x = o.pre ? P(o.yj) : t.x;
+        jac[i].x = o.pre ? P(o.y[npass]) : t.x;
         jac[i].v = P(o.v);
         jac[i].p = o.pre ? o.q_ : o.p_;
         jac[i].q = o.q_;
@@ -646,31 +648,56 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     run_jacobi(c, jac);
     run_select(c, sel);
 
-    // ---- preconditioned tasks: U_X = Q1 [W[:, perm] / σ ; 0],  V_X = Q2 V_J[:, perm]
-    std::vector<GatherItem> pre_g;
-    std::vector<QrApplyJob> app;
+    // ---- preconditioned tasks.  The Jacobi gives Y V_J = W with columns of norm σ;
+    // with Wn = W / σ and R_{j-1} = R_j^H Q_j^H unrolled:
+    //   npass even:  X = (Q1 Q3 .. Q_{n-1}) Y (Q2 Q4 .. Q_n)^H  ->  U_X = Q1 Q3 .. Wn,   V_X = Q2 Q4 .. V_J
+    //   npass odd:   X = (Q1 Q3 .. Q_n) Y^H (Q2 Q4 .. Q_{n-1})^H ->  U_X = Q1 Q3 .. V_J,  V_X = Q2 Q4 .. Wn
+    // (columns in sorted order; the factor next to the source is applied first).
+    const bool even = npass % 2 == 0;
+    std::vector<int> uq, vq;  // factors to apply on each side, innermost first
+    for (int j = npass; j >= 1; --j) (j % 2 ? uq : vq).push_back(j);
+    std::vector<GatherItem> pre_g, to_ux;
     for (size_t i = 0; i < nt; ++i) {
         const Off& o = off[i];
         if (!o.pre) continue;
-        launch_fill(reinterpret_cast<double*>(P(o.ux)), 2 * (size_t)o.p_ * std::max(o.k_, 1), 0.0, c->stream);
+        const size_t kk = std::max(o.k_, 1);
+        launch_fill(reinterpret_cast<double*>(P(o.ux)), 2 * (size_t)o.p_ * kk, 0.0, c->stream);
+        if (npass >= 3) launch_fill(reinterpret_cast<double*>(P(o.tmp)), 2 * (size_t)o.q_ * kk, 0.0, c->stream);
         GatherItem g{};
-        g.src = jac[i].x; g.ld_src = o.q_; g.rows = o.q_; g.perm = sel[i].perm; g.sigma_sorted = sel[i].sigma_sorted;
-        g.divide_sigma = 1; g.kept = sel[i].kept; g.rowscale_mod = 1; g.rowscale_div = 1;
-        g.out = P(o.ux); g.conj_transpose = 0; g.max_cols = o.k_; g.ld_out = o.p_;
+        g.src = even ? jac[i].x : jac[i].v; g.ld_src = o.q_; g.rows = o.q_; g.perm = sel[i].perm;
+        g.sigma_sorted = sel[i].sigma_sorted;
+        g.divide_sigma = even ? 1 : 0; g.kept = sel[i].kept; g.rowscale_mod = 1; g.rowscale_div = 1;
+        g.conj_transpose = 0; g.max_cols = o.k_;
+        if (uq.size() > 1) {  // the q × q factors act on a q-row copy first
+            g.out = P(o.tmp); g.ld_out = o.q_;
+            GatherItem m = g;
+            m.src = P(o.tmp); m.perm = nullptr; m.divide_sigma = 0; m.out = P(o.ux); m.ld_out = o.p_;
+            to_ux.push_back(m);
+        } else {
+            g.out = P(o.ux); g.ld_out = o.p_;
+        }
         pre_g.push_back(g);
         GatherItem h{};
-        h.src = jac[i].v; h.ld_src = o.q_; h.rows = o.q_; h.perm = sel[i].perm; h.sigma_sorted = sel[i].sigma_sorted;
-        h.divide_sigma = 0; h.kept = sel[i].kept; h.rowscale_mod = 1; h.rowscale_div = 1;
+        h.src = even ? jac[i].v : jac[i].x; h.ld_src = o.q_; h.rows = o.q_; h.perm = sel[i].perm;
+        h.sigma_sorted = sel[i].sigma_sorted;
+        h.divide_sigma = even ? 0 : 1; h.kept = sel[i].kept; h.rowscale_mod = 1; h.rowscale_div = 1;
         h.out = P(o.vx); h.conj_transpose = 0; h.max_cols = o.k_;
         pre_g.push_back(h);
     }
     run_gather(c, pre_g);
-    for (size_t i = 0; i < nt; ++i) {
-        if (!off[i].pre) continue;
-        app.push_back(QrApplyJob{&qr1[i], P(off[i].ux), off[i].k_});
-        app.push_back(QrApplyJob{&qr2[i], P(off[i].vx), off[i].k_});
+    const size_t stages = std::max(uq.size(), vq.size());
+    for (size_t r = 0; r < stages; ++r) {
+        const bool u_now = r < uq.size(), v_now = r < vq.size();
+        if (u_now && uq[r] == 1 && uq.size() > 1) run_gather(c, to_ux);
+        std::vector<QrApplyJob> app;
+        for (size_t i = 0; i < nt; ++i) {
+            const Off& o = off[i];
+            if (!o.pre) continue;
+            if (u_now) app.push_back(QrApplyJob{&qr[i][uq[r]], P(uq[r] == 1 ? o.ux : o.tmp), o.k_});
+            if (v_now) app.push_back(QrApplyJob{&qr[i][vq[r]], P(o.vx), o.k_});
+        }
+        run_qr_apply_q(c, app);
     }
-    run_qr_apply_q(c, app);
 
     // ---- outputs
     std::vector<GatherItem> gat;

@@ -405,7 +405,9 @@ def test_workload2_full_size_time_reversal(qh):
     forward, then the same layers in reverse order with the inverse gates.  The
     round trip returns to the initial product state up to the truncation: the
     infidelity is bounded by the summed Eckart-Young errors of all cuts (each cut
-    moves the state by at most its discarded norm; the gates are unitary)."""
+    moves the state by at most its discarded norm; the gates are unitary).  The
+    update does not keep the chain in canonical form, so a cut can move the norm
+    either way: |1 - fidelity| is what the bound holds for."""
     arrays, layers, trunc, _ = wl.config2(steps=10)
     psi = qh.from_internal(arrays)
     psi0 = psi.copy()
@@ -422,6 +424,5 @@ def test_workload2_full_size_time_reversal(qh):
         disc += sum(d)
     fid = abs(qh.overlap(psi, psi0))
     print(f"return fidelity {fid:.15f}, summed discarded norm {disc:.3e}")
-    assert fid <= 1.0 + 1e-10
-    assert 1.0 - fid <= disc + 1e-10
-    assert 1.0 - fid < 1e-5
+    assert abs(1.0 - fid) <= disc + 1e-10
+    assert abs(1.0 - fid) < 1e-5
