@@ -28,6 +28,9 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
 // formulation); naa / nbb are the new diagonal.  No rotation (s = 0) when the
 // pair is already orthogonal, |g_ab|^2 <= tol2·g_aa·g_bb, or a column is below
 // the deflation floor.
+// With d = g_bb - g_aa and h = sqrt(d^2 + 4|g_ab|^2) (= 2|g_ab| sqrt(1 + zeta^2)):
+// |t| = 2|g_ab| / (|d| + h) and c^2 = 1/(1 + t^2) = (|d| + h) / (2h), so the two
+// square roots are independent and the dependent chain is sqrt, divide, sqrt.
 struct JRot {
     double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = 0.0, nbb = 0.0;
 };
@@ -39,12 +42,16 @@ __device__ __forceinline__ JRot jacobi_rot(double gaa, double gbb, double2 gab, 
     r.naa = gaa;
     r.nbb = gbb;
     if (needs_rotation(gaa, gbb, gab, tol2, gfloor)) {
-        const double mag = sqrt(cabs2(gab));
-        const double rmag = 1.0 / mag;
-        const double zeta = (gbb - gaa) * (0.5 * rmag);
-        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        r.c = 1.0 / sqrt(1.0 + t * t);
+        const double g2 = cabs2(gab);
+        const double d = gbb - gaa;
+        const double mag = sqrt(g2);
+        const double h = sqrt(fma(d, d, 4.0 * g2));
+        const double den = fabs(d) + h;
+        const double tabs = (2.0 * mag) / den;
+        const double t = d >= 0.0 ? tabs : -tabs;
+        r.c = sqrt(den / (2.0 * h));
         r.s = r.c * t;
+        const double rmag = 1.0 / mag;
         r.ex = gab.x * rmag;
         r.ey = gab.y * rmag;
         r.naa = gaa - t * mag;
@@ -86,6 +93,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     double2 (*sX)[CH + 1] = reinterpret_cast<double2 (*)[CH + 1]>(sXraw);
     __shared__ int sCol[BC];
     __shared__ int sPerm[BC];
+    __shared__ int sSlot[BC], sOrd[BC];
+    __shared__ double sKey[BC];
     __shared__ int sFlag;
 
     const int tid = threadIdx.x;
@@ -293,25 +302,43 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     // first sweeps only: afterwards the plain cyclic ordering finishes with its usual
     // quadratic convergence (sorting in a parallel ordering can lock into a cycle).
     if (sort) {
-    if (tid == 0) {
-        int slot[BC], src[BC];
-        int nv = 0;
+    // Rank sort, one thread per column: the valid positions are taken in slot order
+    // (the block with the lower index first) and position k of that order receives
+    // the column of the k-th largest eigenvalue, ties kept in slot order (a stable
+    // descending sort).  Invalid positions keep e_j.
+    {
         const int first = (bi < bj) ? 0 : B;
-        for (int h = 0; h < 2; ++h) {
-            const int base = (h == 0) ? first : B - first;
-            for (int j = 0; j < B; ++j)
-                if (sCol[base + j] >= 0) slot[nv++] = base + j;
+        if (tid < BC) {
+            int o = -1;
+            double key = 0.0;
+            if (sCol[tid] >= 0) {
+                const bool in_first = tid >= first && tid < first + B;
+                int cnt = 0;
+                if (!in_first)
+                    for (int u = first; u < first + B; ++u) cnt += (sCol[u] >= 0) ? 1 : 0;
+                for (int u = in_first ? first : B - first; u < tid; ++u) cnt += (sCol[u] >= 0) ? 1 : 0;
+                o = cnt;
+                sSlot[o] = tid;
+                const double x = sG[tid][tid].x;
+                key = (x == x) ? x : -INFINITY;  // a total order even with NaN
+            } else {
+                sPerm[tid] = tid;
+            }
+            sOrd[tid] = o;
+            sKey[tid] = key;
         }
-        for (int k = 0; k < nv; ++k) src[k] = slot[k];
-        for (int k = 1; k < nv; ++k) {  // insertion sort by eigenvalue, descending (stable)
-            const int cur = src[k];
-            const double ev = sG[cur][cur].x;
-            int m = k - 1;
-            while (m >= 0 && sG[src[m]][src[m]].x < ev) { src[m + 1] = src[m]; --m; }
-            src[m + 1] = cur;
+        __syncthreads();
+        if (tid < BC && sOrd[tid] >= 0) {
+            const int o = sOrd[tid];
+            const double key = sKey[tid];
+            int rank = 0;
+            for (int u = 0; u < BC; ++u) {
+                const int ou = sOrd[u];
+                const double ku = sKey[u];
+                rank += (ou >= 0 && (ku > key || (ku == key && ou < o))) ? 1 : 0;
+            }
+            sPerm[sSlot[rank]] = tid;
         }
-        for (int j = 0; j < BC; ++j) sPerm[j] = j;
-        for (int k = 0; k < nv; ++k) sPerm[slot[k]] = src[k];
     }
     __syncthreads();
     for (int e = tid; e < BC * BC; e += 256) {
