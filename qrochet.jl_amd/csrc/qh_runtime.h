@@ -66,9 +66,9 @@ struct qh_context {
     int jac_block = 16;
     int jac_max_sweeps = 40;
     int jac_sort_sweeps = 2;  // sweeps that order eigenpairs by size (then plain cyclic)
-    // QR passes preconditioning SVDs with more than one block pair (0..3): X = Q1 R1,
+    // QR passes preconditioning SVDs with more than one block pair (0..4): X = Q1 R1,
     // R_{j-1}^H = Q_j R_j, Jacobi on R_n^H
-    int jac_precondition = 2;
+    int jac_precondition = 3;
     int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
     // device buffers allocated by the entry point running now and not yet owned by
     // an MPS: freed if that call fails, so an error leaks nothing
@@ -183,7 +183,7 @@ struct SvdTask {
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks);
 
 // most QR passes of the Jacobi preconditioner (qh_context::jac_precondition)
-constexpr int QR_MAX_PASSES = 3;
+constexpr int QR_MAX_PASSES = 4;
 // Householder QR of every job in place (R in the upper triangle); reflectors into v (m × n),
 // compact-WY triangles into t (ceil(n / QR_NB) · QR_NB² entries).
 struct QrJob {
