@@ -16,7 +16,6 @@
 namespace qh {
 
 constexpr int QR_TC = 16;  // target columns per apply workgroup
-constexpr int QR_CH = 32;  // rows per LDS chunk
 
 // block-wide sum of K complex values per thread (256 threads); result in out[K]
 __device__ __forceinline__ double2 shfl_xor2(double2 a, int off) {
@@ -362,10 +361,15 @@ __global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ ite
 }
 
 // Y[j0:, cols] <- (I - V T' V^H) Y[j0:, cols],  T' = T^H (adjoint) or T
+// One workgroup per (item, tile of QR_TC columns).  W = V^H Y is accumulated in
+// 4 × 4 register tiles (lane: row subgroup g, reflector group tk, column group tq)
+// over 64-row chunks staged in LDS, then summed over subgroups and waves; the
+// update Y -= V (T' W) gives each thread 2 rows × 4 columns of a 128-row chunk.
 __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
                                                   int j0, int adjoint) {
-    __shared__ double2 sV[QR_CH][QR_NB + 1];
-    __shared__ double2 sY[QR_CH][QR_TC + 1];
+    static_assert(QR_TC == QR_NB && QR_NB == 16, "tiles are 16 reflectors × 16 columns");
+    constexpr int RC = 64;  // rows per phase-1 chunk (phase 3: 2·RC)
+    __shared__ double2 sBuf[2 * RC][QR_NB + 1];
     __shared__ double2 sW[QR_NB][QR_TC + 1];
     __shared__ double2 sT[QR_NB][QR_NB + 1];
     const int2 tt = tiles[blockIdx.x];
@@ -382,28 +386,63 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
         const int i = e % QR_NB, j = e / QR_NB;
         sT[i][j] = t[i + QR_NB * j];
     }
-    // phase 1: W = V^H Y   (thread: a = tid / 16, col = tid % 16)
-    const int wa = tid / QR_TC, wc = tid % QR_TC;
-    double2 w = cz();
-    for (int r0 = j0; r0 < m; r0 += QR_CH) {
-        for (int e = tid; e < QR_CH * QR_NB; e += 256) {
-            const int rr = e % QR_CH, k = e / QR_CH;
+    // ---- phase 1: W = V^H Y
+    const int lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, tk = (lane >> 2) & 3, tq = lane & 3;
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = cz();
+    for (int r0 = j0; r0 < m; r0 += RC) {
+        for (int e = tid; e < RC * QR_NB; e += 256) {
+            const int rr = e % RC, k = e / RC;
             const int r = r0 + rr;
-            sV[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)it.m * (j0 + k)] : cz();
-        }
-        for (int e = tid; e < QR_CH * QR_TC; e += 256) {
-            const int rr = e % QR_CH, k = e / QR_CH;
-            const int r = r0 + rr;
-            sY[rr][k] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
+            sBuf[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
+            sBuf[RC + rr][k] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
         }
         __syncthreads();
-#pragma unroll 8
-        for (int rr = 0; rr < QR_CH; ++rr) w = cfmac(sV[rr][wa], sY[rr][wc], w);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int rr = 16 * wv + g + 4 * i;
+            double2 vv[4], yy[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                vv[a] = sBuf[rr][4 * tk + a];
+                yy[a] = sBuf[RC + rr][4 * tq + a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = cfmac(vv[a], yy[b], acc[a][b]);
+        }
         __syncthreads();
     }
-    sW[wa][wc] = w;
+    // sum over the four row subgroups of a wave, then over the waves (through sBuf)
+    double2* sPart = &sBuf[0][0];  // 4 × (QR_NB · QR_TC) values, fits in sBuf
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = cadd(acc[a][b], shfl_xor2(acc[a][b], 16));
+            acc[a][b] = cadd(acc[a][b], shfl_xor2(acc[a][b], 32));
+        }
+    if (g == 0) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) sPart[wv * QR_NB * QR_TC + (4 * tk + a) * QR_TC + 4 * tq + b] = acc[a][b];
+    }
     __syncthreads();
-    // phase 2: W2 = T' W
+    const int wa = tid / QR_TC, wc = tid % QR_TC;
+    {
+        const int e = wa * QR_TC + wc;
+        const double2 w = cadd(cadd(sPart[e], sPart[QR_NB * QR_TC + e]),
+                               cadd(sPart[2 * QR_NB * QR_TC + e], sPart[3 * QR_NB * QR_TC + e]));
+        sW[wa][wc] = w;
+    }
+    __syncthreads();
+    // ---- phase 2: W2 = T' W
     double2 w2 = cz();
     for (int b = 0; b < QR_NB; ++b) {
         const double2 tab = adjoint ? cconj(sT[b][wa]) : sT[wa][b];
@@ -412,26 +451,44 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
     __syncthreads();
     sW[wa][wc] = w2;
     __syncthreads();
-    // phase 3: Y -= V W2   (thread: row rr = tid % 32, columns (tid / 32) * 2 + {0,1})
-    const int rr = tid % QR_CH, cg = (tid / QR_CH) * 2;
-    for (int r0 = j0; r0 < m; r0 += QR_CH) {
-        const int r = r0 + rr;
-        if (r < m) {
-            double2 acc0 = cz(), acc1 = cz();
-            for (int k = 0; k < nb; ++k) {
-                const double2 vr = it.v[r + (size_t)it.m * (j0 + k)];
-                acc0 = cfma(vr, sW[k][cg], acc0);
-                acc1 = cfma(vr, sW[k][cg + 1], acc1);
-            }
-            if (cg < tc) {
-                double2& y = it.y[r + (size_t)it.ldy * (c0 + cg)];
-                y = csub(y, acc0);
-            }
-            if (cg + 1 < tc) {
-                double2& y = it.y[r + (size_t)it.ldy * (c0 + cg + 1)];
-                y = csub(y, acc1);
+    // ---- phase 3: Y -= V W2   (thread: rows rt, rt + 64 of a chunk, columns 4·cg .. 4·cg + 3)
+    const int rt = tid & 63, cg = tid >> 6;
+    for (int r0 = j0; r0 < m; r0 += 2 * RC) {
+        for (int e = tid; e < 2 * RC * QR_NB; e += 256) {
+            const int rr = e % (2 * RC), k = e / (2 * RC);
+            const int r = r0 + rr;
+            sBuf[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
+        }
+        __syncthreads();
+        double2 o[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) o[a][b] = cz();
+#pragma unroll 4
+        for (int k = 0; k < QR_NB; ++k) {
+            const double2 v0 = sBuf[rt][k], v1 = sBuf[rt + RC][k];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double2 w = sW[k][4 * cg + b];
+                o[0][b] = cfma(v0, w, o[0][b]);
+                o[1][b] = cfma(v1, w, o[1][b]);
             }
         }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int r = r0 + rt + RC * a;
+            if (r >= m) continue;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int c = 4 * cg + b;
+                if (c < tc) {
+                    double2& y = it.y[r + (size_t)it.ldy * (c0 + c)];
+                    y = csub(y, o[a][b]);
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
