@@ -360,25 +360,24 @@ __global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ ite
     else qr_panel_global(it, j0, sh);
 }
 
-// Y[j0:, cols] <- (I - V T' V^H) Y[j0:, cols],  T' = T^H (adjoint) or T
-// One workgroup per (item, tile of QR_TC columns).  W = V^H Y is accumulated in
-// 4 × 4 register tiles (lane: row subgroup g, reflector group tk, column group tq)
-// over 64-row chunks staged in LDS, then summed over subgroups and waves; the
-// update Y -= V (T' W) gives each thread 2 rows × 4 columns of a 128-row chunk.
-__global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
-                                                  int j0, int adjoint) {
+// Y[j0:, c0 : c0 + tc] <- (I - V T' V^H) Y[j0:, c0 : c0 + tc],  T' = T^H (adjoint) or T,
+// V and T of the panel of nb reflectors at j0.  W = V^H Y is accumulated in 4 × 4
+// register tiles (lane: row subgroup g, reflector group tk, column group tq) over
+// 64-row chunks staged in LDS, then summed over subgroups and waves; the update
+// Y -= V (T' W) gives each thread 2 rows × 4 columns of a 128-row chunk.
+struct ApplyShared {
+    static constexpr int RC = 64;  // rows per phase-1 chunk (phase 3: 2·RC)
+    double2 sBuf[2 * RC][QR_NB + 1];
+    double2 sW[QR_NB][QR_TC + 1];
+    double2 sT[QR_NB][QR_NB + 1];
+};
+__device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, int tc, int j0, int nb, int adjoint,
+                                               ApplyShared& sh) {
     static_assert(QR_TC == QR_NB && QR_NB == 16, "tiles are 16 reflectors × 16 columns");
-    constexpr int RC = 64;  // rows per phase-1 chunk (phase 3: 2·RC)
-    __shared__ double2 sBuf[2 * RC][QR_NB + 1];
-    __shared__ double2 sW[QR_NB][QR_TC + 1];
-    __shared__ double2 sT[QR_NB][QR_NB + 1];
-    const int2 tt = tiles[blockIdx.x];
-    const QrApplyItem it = items[tt.x];
-    const int nb = min(QR_NB, it.nrefl - j0);
-    if (nb <= 0) return;
-    const int c0 = (it.trailing ? j0 + nb : 0) + tt.y * QR_TC;
-    const int tc = min(QR_TC, it.ncols - c0);
-    if (tc <= 0) return;
+    constexpr int RC = ApplyShared::RC;
+    auto& sBuf = sh.sBuf;
+    auto& sW = sh.sW;
+    auto& sT = sh.sT;
     const int tid = threadIdx.x;
     const int m = it.m;
     const double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
@@ -394,14 +393,29 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = cz();
-    for (int r0 = j0; r0 < m; r0 += RC) {
-        for (int e = tid; e < RC * QR_NB; e += 256) {
+    // chunk r0 + RC is fetched into registers while chunk r0 is reduced from LDS
+    constexpr int E1 = RC * QR_NB / 256;
+    double2 pv[E1], py[E1];
+    auto fetch1 = [&](int r0) {
+#pragma unroll
+        for (int s2 = 0; s2 < E1; ++s2) {
+            const int e = tid + 256 * s2;
             const int rr = e % RC, k = e / RC;
             const int r = r0 + rr;
-            sBuf[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
-            sBuf[RC + rr][k] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
+            pv[s2] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
+            py[s2] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
+        }
+    };
+    fetch1(j0);
+    for (int r0 = j0; r0 < m; r0 += RC) {
+#pragma unroll
+        for (int s2 = 0; s2 < E1; ++s2) {
+            const int e = tid + 256 * s2;
+            sBuf[e % RC][e / RC] = pv[s2];
+            sBuf[RC + e % RC][e / RC] = py[s2];
         }
         __syncthreads();
+        if (r0 + RC < m) fetch1(r0 + RC);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int rr = 16 * wv + g + 4 * i;
@@ -453,13 +467,41 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
     __syncthreads();
     // ---- phase 3: Y -= V W2   (thread: rows rt, rt + 64 of a chunk, columns 4·cg .. 4·cg + 3)
     const int rt = tid & 63, cg = tid >> 6;
-    for (int r0 = j0; r0 < m; r0 += 2 * RC) {
-        for (int e = tid; e < 2 * RC * QR_NB; e += 256) {
+    // V of chunk r0 + 2·RC and this thread's Y entries of it are fetched during chunk r0
+    constexpr int E3 = 2 * RC * QR_NB / 256;
+    double2 qv[E3], qy[2][4];
+    auto fetch3 = [&](int r0) {
+#pragma unroll
+        for (int s2 = 0; s2 < E3; ++s2) {
+            const int e = tid + 256 * s2;
             const int rr = e % (2 * RC), k = e / (2 * RC);
             const int r = r0 + rr;
-            sBuf[rr][k] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
+            qv[s2] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
         }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int r = r0 + rt + RC * a;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int c = 4 * cg + b;
+                qy[a][b] = (r < m && c < tc) ? it.y[r + (size_t)it.ldy * (c0 + c)] : cz();
+            }
+        }
+    };
+    fetch3(j0);
+    for (int r0 = j0; r0 < m; r0 += 2 * RC) {
+#pragma unroll
+        for (int s2 = 0; s2 < E3; ++s2) {
+            const int e = tid + 256 * s2;
+            sBuf[e % (2 * RC)][e / (2 * RC)] = qv[s2];
+        }
+        double2 yc[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) yc[a][b] = qy[a][b];
         __syncthreads();
+        if (r0 + 2 * RC < m) fetch3(r0 + 2 * RC);
         double2 o[2][4];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -482,12 +524,38 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int c = 4 * cg + b;
-                if (c < tc) {
-                    double2& y = it.y[r + (size_t)it.ldy * (c0 + c)];
-                    y = csub(y, o[a][b]);
-                }
+                if (c < tc) it.y[r + (size_t)it.ldy * (c0 + c)] = csub(yc[a][b], o[a][b]);
             }
         }
+        __syncthreads();
+    }
+}
+
+// one block reflector (panel j0) on one column tile per workgroup
+__global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
+                                                  int j0, int adjoint) {
+    __shared__ ApplyShared sh;
+    const int2 tt = tiles[blockIdx.x];
+    const QrApplyItem it = items[tt.x];
+    const int nb = min(QR_NB, it.nrefl - j0);
+    if (nb <= 0) return;
+    const int c0 = (it.trailing ? j0 + nb : 0) + tt.y * QR_TC;
+    const int tc = min(QR_TC, it.ncols - c0);
+    if (tc <= 0) return;
+    qr_apply_panel(it, c0, tc, j0, nb, adjoint, sh);
+}
+
+// Y <- Q Y with every block reflector of the factorisation, last panel first: the
+// column tiles are independent, so one workgroup walks all panels of its tile.
+__global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles) {
+    __shared__ ApplyShared sh;
+    const int2 tt = tiles[blockIdx.x];
+    const QrApplyItem it = items[tt.x];
+    const int c0 = tt.y * QR_TC;
+    const int tc = min(QR_TC, it.ncols - c0);
+    if (tc <= 0 || it.nrefl <= 0) return;
+    for (int j0 = ((it.nrefl - 1) / QR_NB) * QR_NB; j0 >= 0; j0 -= QR_NB) {
+        qr_apply_panel(it, c0, tc, j0, min(QR_NB, it.nrefl - j0), 0, sh);
         __syncthreads();
     }
 }
@@ -499,6 +567,10 @@ void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
 void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s) {
     if (ntiles <= 0) return;
     k_qr_apply<<<ntiles, 256, 0, s>>>(d_items, d_tiles, j0, adjoint);
+}
+void launch_qr_form(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_qr_form<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 int qr_tile_cols() { return QR_TC; }
 

@@ -538,12 +538,12 @@ void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs) {
         nmax = std::max(nmax, q.n);
         for (int k = 0; k < (jobs[i].ncols + tc - 1) / tc; ++k) tiles.push_back(make_int2((int)i, k));
     }
-    if (tiles.empty()) return;
+    if (tiles.empty() || nmax <= 0) return;
     QrApplyItem* da = upload(c, app);
     int2* dt = upload(c, tiles);
     Timed tm(c, "qr_apply");
-    const int last = ((nmax - 1) / QR_NB) * QR_NB;
-    for (int j0 = last; j0 >= 0; j0 -= QR_NB) launch_qr_apply(da, dt, (int)tiles.size(), j0, 0, c->stream);
+    // column tiles are independent: one launch, each workgroup applies every panel
+    launch_qr_form(da, dt, (int)tiles.size(), c->stream);
 }
 
 int SvdTask::kmax() const {
