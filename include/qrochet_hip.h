@@ -83,6 +83,11 @@ int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_m
 int qh_ctx_reset_timing(qh_context* ctx);
 /* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default. */
 int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);
+/* Householder QR passes that precondition the Jacobi SVD of matrices with more
+ * than one block pair (0..4, default 3): X = Q1 R1, R_{j-1}^H = Q_j R_j, Jacobi on
+ * R_n^H.  Changes the rounding of the results only, not their meaning (no
+ * reference counterpart: a tuning knob of this SVD). */
+int qh_ctx_set_qr_passes(qh_context* ctx, int passes);
 /* Device memory of the context's GPU (no reference counterpart: the request loop
  * sizes its admission budget from total_bytes). */
 int qh_ctx_mem_info(qh_context* ctx, size_t* free_bytes, size_t* total_bytes);

@@ -777,7 +777,6 @@ int qh_ctx_create(int device, qh_context** out) {
             const long long v = std::atoll(e);
             if (v >= 256) c->arena_first = (size_t)v;
         }
-        if (const char* e = std::getenv("QH_JACOBI_PRECONDITION")) c->jac_precondition = std::atoi(e);
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t thr = UINT64_MAX;
@@ -852,7 +851,14 @@ int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
         if (block) c->jac_block = block;
         if (max_sweeps > 0) c->jac_max_sweeps = max_sweeps;
         if (const char* e = std::getenv("QH_JACOBI_MAX_INNER")) c->jac_max_inner = std::max(1, std::atoi(e));
-        if (const char* e = std::getenv("QH_JACOBI_PRECONDITION")) c->jac_precondition = std::atoi(e);
+    });
+}
+
+int qh_ctx_set_qr_passes(qh_context* c, int passes) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        if (passes < 0 || passes > QR_MAX_PASSES) throw Error(QH_ERR_ARGUMENT, "QR passes must be in 0..4");
+        c->jac_precondition = passes;
     });
 }
 

@@ -52,6 +52,7 @@ SIGNATURES = {
     "qh_ctx_get_timing": (C.c_int, [_ctx_p, C.c_char_p, _dp, C.POINTER(C.c_int64), _dp, _dp]),
     "qh_ctx_reset_timing": (C.c_int, [_ctx_p]),
     "qh_ctx_set_jacobi": (C.c_int, [_ctx_p, C.c_int, C.c_int]),
+    "qh_ctx_set_qr_passes": (C.c_int, [_ctx_p, C.c_int]),
     "qh_ctx_mem_info": (C.c_int, [_ctx_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "qh_mps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_mps_p)]),
     "qh_pmps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mps_p)]),

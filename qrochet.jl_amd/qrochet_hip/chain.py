@@ -174,6 +174,10 @@ class Context:
     def set_jacobi(self, block=0, max_sweeps=0):
         self.check(L.load().qh_ctx_set_jacobi(self.handle, block, max_sweeps))
 
+    def set_qr_passes(self, passes: int):
+        """Householder QR passes preconditioning the Jacobi SVD (0..4, default 3)."""
+        self.check(L.load().qh_ctx_set_qr_passes(self.handle, int(passes)))
+
     def mem_info(self):
         """(free, total) bytes of the device."""
         f = C.c_size_t()

@@ -426,3 +426,38 @@ def test_workload2_full_size_time_reversal(qh):
     print(f"return fidelity {fid:.15f}, summed discarded norm {disc:.3e}")
     assert abs(1.0 - fid) <= disc + 1e-10
     assert abs(1.0 - fid) < 1e-5
+
+
+@pytest.mark.parametrize("passes", [0, 1, 2, 3, 4])
+def test_qr_preconditioning_passes(qh, passes):
+    """Every number of Householder passes preconditioning the Jacobi SVD (X = Q1 R1,
+    R_{j-1}^H = Q_j R_j, Jacobi on R_n^H; the odd factors end on the left singular
+    vectors, the even ones on the right) evolves like the dense state vector:
+    untruncated brickwork layers on 14 sites whose theta grows to 256 x 256 (the
+    preconditioned path runs from 64 columns up), and the new sites stay isometries."""
+    rng = np.random.default_rng(17)
+    n = 14
+    arrays = wl.left_canonize_normalize(wl.random_mps_arrays(rng, n, 2, wl.max_profile(n, 2, 64)))
+    ctx = qh.Context()
+    ctx.set_qr_passes(passes)
+    psi = qh.from_internal(arrays, context=ctx)
+    v = oc.statevector(oc.Chain([a.copy() for a in arrays]))
+    for layer in range(3):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            qh.evolve_(psi, qh.gate(g, b, b + 1))
+            v = sv.apply2(v, g, b)
+    assert max(psi.bond_dims()) == 128
+    got = oc.statevector(oc.Chain(psi.internal_arrays(), psi.lambdas()))
+    err = np.max(np.abs(got - v))
+    print(f"passes {passes}: max |psi - psi_exact| = {err:.3e}")
+    assert err < 1e-10
+    b = n // 2
+    left = psi.internal_arrays()[b - 1]
+    lm = left.reshape(-1, left.shape[2], order="F")
+    assert np.max(np.abs(lm.conj().T @ lm - np.eye(lm.shape[1]))) < 1e-12
+    right = psi.internal_arrays()[b]
+    rm = right.reshape(right.shape[0], -1, order="F")
+    assert np.max(np.abs(rm @ rm.conj().T - np.eye(rm.shape[0]))) < 1e-12
+    with pytest.raises(qh.ArgumentError):
+        ctx.set_qr_passes(5)
