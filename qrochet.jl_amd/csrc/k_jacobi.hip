@@ -224,8 +224,11 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     // A round applies NP disjoint rotations.  Thread (pb, qb) owns the 2×2 block of G
     // at (row pair pb, column pair qb) and the 2×2 block of Q at rows {2pb, 2pb+1} ×
     // column pair qb.  It derives both rotations from the round's starting G, forms
-    // J_pb^H (G_blk J_qb) and Q_blk J_qb in registers and writes them back after one
-    // barrier: two barriers per round and no serial phase.
+    // J_pb^H (G_blk J_qb) and Q_blk J_qb in registers and writes them back: G into the
+    // other of two buffers (sG and the idle staging area, every entry every round), so
+    // one barrier per round separates a round's reads from the next round's writes;
+    // Q in place, since rows 2pb, 2pb+1 belong to the NP threads of one wave (whose
+    // LDS operations execute in order).
     for (int e = tid; e < BC * BC; e += 256) {
         const int i = e / BC, j = e % BC;
         sQ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
@@ -237,35 +240,39 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     static_assert((NP * NP) % 64 == 0, "block owners must fill whole waves");
     const bool owner = tid < NP * NP;
     const int pb = tid / NP, qb = tid % NP;
+    static_assert(SX_ELEMS >= BC * LP, "the staging area holds the second G buffer");
+    double2 (*gA)[LP] = sG;                                       // G at the start of the round
+    double2 (*gB)[LP] = reinterpret_cast<double2 (*)[LP]>(sXraw);  // G after the round
 #pragma unroll 1
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         if (tid == 0) sFlag = 0;
         __syncthreads();
 #pragma unroll 1
         for (int rnd = 0; rnd < BC - 1; ++rnd) {
-            int ap = 0, bp = 0, aq = 0, bq = 0;
-            double2 g00, g01, g10, g11, q00, q01, q10, q11;
-            JRot jp, jq;
             if (owner) {
+                int ap, bp, aq, bq;
                 rr_pair(rnd, pb, BC, ap, bp);
                 rr_pair(rnd, qb, BC, aq, bq);
                 // each lane solves its column pair; the row pair's rotation comes from
                 // the lane of the same wave that owns diagonal block (pb, pb)
-                jq = jacobi_rot(sG[aq][aq].x, sG[bq][bq].x, sG[aq][bq], tol2, gfloor);
+                const JRot jq = jacobi_rot(gA[aq][aq].x, gA[bq][bq].x, gA[aq][bq], tol2, gfloor);
                 // (whole waves are owners or not, so every lane of the wave shuffles)
                 const int src = (pb % (64 / NP)) * NP + pb;
+                JRot jp;
                 jp.c = __shfl(jq.c, src, 64);
                 jp.s = __shfl(jq.s, src, 64);
                 jp.ex = __shfl(jq.ex, src, 64);
                 jp.ey = __shfl(jq.ey, src, 64);
                 if (pb == qb) jp = jq;
-                g00 = sG[ap][aq]; g01 = sG[ap][bq]; g10 = sG[bp][aq]; g11 = sG[bp][bq];
-                q00 = sQ[2 * pb][aq]; q01 = sQ[2 * pb][bq]; q10 = sQ[2 * pb + 1][aq]; q11 = sQ[2 * pb + 1][bq];
+                double2 g00 = gA[ap][aq], g01 = gA[ap][bq], g10 = gA[bp][aq], g11 = gA[bp][bq];
                 if (jq.s != 0.0) {  // columns:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
+                    double2 q00 = sQ[2 * pb][aq], q01 = sQ[2 * pb][bq];
+                    double2 q10 = sQ[2 * pb + 1][aq], q11 = sQ[2 * pb + 1][bq];
                     rot_cols(g00, g01, jq);
                     rot_cols(g10, g11, jq);
                     rot_cols(q00, q01, jq);
                     rot_cols(q10, q11, jq);
+                    sQ[2 * pb][aq] = q00; sQ[2 * pb][bq] = q01; sQ[2 * pb + 1][aq] = q10; sQ[2 * pb + 1][bq] = q11;
                 }
                 if (jp.s != 0.0) {
                     if (pb == qb) {  // the diagonal block becomes diag(λ_a, λ_b)
@@ -278,18 +285,13 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
                         rot_rows(g01, g11, jp);
                     }
                 }
+                gB[ap][aq] = g00; gB[ap][bq] = g01; gB[bp][aq] = g10; gB[bp][bq] = g11;
                 if (pb == qb && jp.s != 0.0) sFlag = 1;
             }
             __syncthreads();
-            if (owner) {
-                if (jp.s != 0.0 || jq.s != 0.0) {
-                    sG[ap][aq] = g00; sG[ap][bq] = g01; sG[bp][aq] = g10; sG[bp][bq] = g11;
-                }
-                if (jq.s != 0.0) {
-                    sQ[2 * pb][aq] = q00; sQ[2 * pb][bq] = q01; sQ[2 * pb + 1][aq] = q10; sQ[2 * pb + 1][bq] = q11;
-                }
-            }
-            __syncthreads();
+            double2 (*gt)[LP] = gA;
+            gA = gB;
+            gB = gt;
         }
         const int any = sFlag;
         __syncthreads();
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
                 for (int u = in_first ? first : B - first; u < tid; ++u) cnt += (sCol[u] >= 0) ? 1 : 0;
                 o = cnt;
                 sSlot[o] = tid;
-                const double x = sG[tid][tid].x;
+                const double x = gA[tid][tid].x;  // the eigensolver's final G
                 key = (x == x) ? x : -INFINITY;  // a total order even with NaN
             } else {
                 sPerm[tid] = tid;
