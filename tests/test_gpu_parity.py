@@ -461,3 +461,30 @@ def test_qr_preconditioning_passes(qh, passes):
     assert np.max(np.abs(rm @ rm.conj().T - np.eye(rm.shape[0]))) < 1e-12
     with pytest.raises(qh.ArgumentError):
         ctx.set_qr_passes(5)
+
+
+@pytest.mark.parametrize("block", [8, 16])
+def test_jacobi_block_sizes(qh, block):
+    """Both column-block widths of the Jacobi tournament (8 and 16 columns per block,
+    `qh_ctx_set_jacobi`) evolve like the dense state vector: untruncated brickwork
+    layers on 12 sites (theta up to 128 x 128: 16 blocks of 8 or 8 blocks of 16,
+    QR-preconditioned, and the small thetas at the chain ends with one block pair)."""
+    rng = np.random.default_rng(23)
+    n = 12
+    arrays = wl.left_canonize_normalize(wl.random_mps_arrays(rng, n, 2, wl.max_profile(n, 2, 32)))
+    ctx = qh.Context()
+    ctx.set_jacobi(block=block)
+    psi = qh.from_internal(arrays, context=ctx)
+    v = oc.statevector(oc.Chain([a.copy() for a in arrays]))
+    for layer in range(4):
+        for b in wl.brickwork_bonds(n, layer):
+            g = wl.haar_gate4(rng)
+            qh.evolve_(psi, qh.gate(g, b, b + 1))
+            v = sv.apply2(v, g, b)
+    assert max(psi.bond_dims()) == 64
+    err = np.max(np.abs(oc.statevector(oc.Chain(psi.internal_arrays(), psi.lambdas())) - v))
+    print(f"block {block}: max |psi - psi_exact| = {err:.3e}")
+    assert err < 1e-10
+    for lg in psi.lambdas():
+        if lg is not None:
+            assert np.all(np.diff(lg) <= 0)  # singular values sorted, descending
