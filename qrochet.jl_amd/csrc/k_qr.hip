@@ -560,6 +560,36 @@ __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__
     }
 }
 
+// Look-ahead step of the factorisation: workgroups [0, count) factor panel j0 + QR_NB
+// of every item (its columns already carry panel j0's update), the others apply
+// panel j0's block reflector to the trailing tiles beyond that panel.  The two
+// parts touch disjoint columns and disjoint reflector storage, so they run in one
+// launch and the panel's latency hides behind the trailing update.
+__global__ __launch_bounds__(256) void k_qr_step(const QrItem* __restrict__ items, int count,
+                                                 const QrApplyItem* __restrict__ app, const int2* __restrict__ tiles,
+                                                 int j0) {
+    __shared__ union {
+        PanelShared p;
+        ApplyShared a;
+    } sh;
+    if ((int)blockIdx.x < count) {
+        const QrItem it = items[blockIdx.x];
+        const int jn = j0 + QR_NB;
+        if (jn >= it.n) return;
+        if (it.m - jn <= QR_REG_ROWS) qr_panel_regs(it, jn, sh.p);
+        else qr_panel_global(it, jn, sh.p);
+        return;
+    }
+    const int2 tt = tiles[blockIdx.x - count];
+    const QrApplyItem ai = app[tt.x];
+    const int nb = min(QR_NB, ai.nrefl - j0);
+    if (nb <= 0) return;
+    const int c0 = j0 + nb + tt.y * QR_TC;
+    const int tc = min(QR_TC, ai.ncols - c0);
+    if (tc <= 0) return;
+    qr_apply_panel(ai, c0, tc, j0, nb, 1, sh.a);
+}
+
 void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
     if (count <= 0) return;
     k_qr_panel<<<count, 256, 0, s>>>(d_items, j0);
@@ -571,6 +601,11 @@ void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles
 void launch_qr_form(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
     if (ntiles <= 0) return;
     k_qr_form<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, const int2* d_tiles, int ntiles,
+                    int j0, hipStream_t s) {
+    if (count + ntiles <= 0) return;
+    k_qr_step<<<count + ntiles, 256, 0, s>>>(d_items, count, d_app, d_tiles, j0);
 }
 int qr_tile_cols() { return QR_TC; }
 

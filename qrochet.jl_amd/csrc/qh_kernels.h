@@ -170,6 +170,10 @@ void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s);
 // apply panel j0's block reflector: adjoint = 1 -> Q_panel^H (factorisation), 0 -> Q_panel
 // tile map: (item, column tile of qr_tile_cols() columns)
 void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s);
+// factorisation look-ahead: panel j0 + QR_NB of every item (workgroups [0, count)) and
+// panel j0's trailing update on the tiles (item, k >= 1) beyond it, in one launch
+void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, const int2* d_tiles, int ntiles,
+                    int j0, hipStream_t s);
 // y <- Q y with all block reflectors of each item (tile map: item, tile of QR_TC columns)
 void launch_qr_form(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 int qr_tile_cols();

@@ -503,7 +503,9 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
     if (jobs.empty()) return;
     std::vector<QrItem> items(jobs.size());
     std::vector<QrApplyItem> app(jobs.size());
-    std::vector<int2> tiles;
+    // trailing tiles of a panel step: tile 0 is the next panel's columns, tiles >= 1
+    // the columns beyond it
+    std::vector<int2> next_tiles, rest_tiles;
     int nmax = 0;
     const int tc = qr_tile_cols();
     double flops = 0;
@@ -513,16 +515,21 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
         app[i] = QrApplyItem{j.v, j.t, j.m, j.n, j.a, j.lda, j.n, 1};
         nmax = std::max(nmax, j.n);
         const int ntile = (std::max(j.n - QR_NB, 0) + tc - 1) / tc;
-        for (int k = 0; k < ntile; ++k) tiles.push_back(make_int2((int)i, k));
+        if (ntile > 0) next_tiles.push_back(make_int2((int)i, 0));
+        for (int k = 1; k < ntile; ++k) rest_tiles.push_back(make_int2((int)i, k));
         flops += 32.0 / 3.0 * j.n * (double)j.n * (3.0 * j.m - j.n) / 2.0;
     }
     QrItem* di = upload(c, items);
     QrApplyItem* da = upload(c, app);
-    int2* dt = upload(c, tiles);
+    int2* dn = upload(c, next_tiles);
+    int2* dr = upload(c, rest_tiles);
     Timed tm(c, "qr", flops, 0.0);
-    for (int j0 = 0; j0 < nmax; j0 += QR_NB) {
-        launch_qr_panel(di, (int)items.size(), j0, c->stream);
-        launch_qr_apply(da, dt, (int)tiles.size(), j0, 1, c->stream);
+    // look-ahead: after panel j0, update only the next panel's columns, then factor
+    // that panel in the same launch as the rest of panel j0's trailing update
+    launch_qr_panel(di, (int)items.size(), 0, c->stream);
+    for (int j0 = 0; j0 + QR_NB < nmax; j0 += QR_NB) {
+        launch_qr_apply(da, dn, (int)next_tiles.size(), j0, 1, c->stream);
+        launch_qr_step(di, (int)items.size(), da, dr, (int)rest_tiles.size(), j0, c->stream);
     }
 }
 
