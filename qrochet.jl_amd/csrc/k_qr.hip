@@ -609,17 +609,19 @@ void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, 
 }
 int qr_tile_cols() { return QR_TC; }
 
-// dst (n × n) = upper triangle of src (ld lds)^H, zeros above the diagonal of dst
-__global__ void k_upper_h(const double2* __restrict__ src, int lds, int n, double2* __restrict__ dst) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * n) return;
-    const int i = (int)(e % n), j = (int)(e / n);  // dst(i, j)
-    dst[e] = (i >= j) ? cconj(src[j + (size_t)lds * i]) : cz();
+// dst (n × n) = upper triangle of src (ld lds)^H, zeros above the diagonal of dst, for
+// a batch (tile map: item, block of 256 entries)
+__global__ void k_upper_h_batch(const UpperHItem* __restrict__ items, const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const UpperHItem it = items[tt.x];
+    const size_t e = (size_t)tt.y * 256 + threadIdx.x;
+    if (e >= (size_t)it.n * it.n) return;
+    const int i = (int)(e % it.n), j = (int)(e / it.n);
+    it.dst[e] = (i >= j) ? cconj(it.src[j + (size_t)it.lds * i]) : cz();
 }
-void launch_upper_h(const cplx* src, int lds, int n, cplx* dst, hipStream_t s) {
-    const size_t t = (size_t)n * n;
-    if (t == 0) return;
-    k_upper_h<<<(int)((t + 255) / 256), 256, 0, s>>>(src, lds, n, dst);
+void launch_upper_h_batch(const UpperHItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_upper_h_batch<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 
 }  // namespace qh

@@ -177,8 +177,14 @@ void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, 
 // y <- Q y with all block reflectors of each item (tile map: item, tile of QR_TC columns)
 void launch_qr_form(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 int qr_tile_cols();
-// dst (n × n) = (upper triangle of src)^H, zero above the diagonal
-void launch_upper_h(const cplx* src, int lds, int n, cplx* dst, hipStream_t s);
+// dst (n × n) = (upper triangle of src)^H, zero above the diagonal, for a batch
+struct UpperHItem {
+    const cplx* src;
+    int lds, n;
+    cplx* dst;
+};
+// tile map: (item, block of 256 entries of the n × n output)
+void launch_upper_h_batch(const UpperHItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // small kernels

@@ -596,11 +596,19 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
                 o.qt[j] = take(tq * sizeof(cplx));
                 o.y[j] = take(q * q * sizeof(cplx));
             }
-            o.ux = take(p * k * sizeof(cplx));
             o.vx = take(q * k * sizeof(cplx));
-            if (npass >= 3) o.tmp = take(q * k * sizeof(cplx));
         }
     }
+    // the buffers that start as zeros (ux and tmp) sit together, cleared by one fill
+    const size_t zero_begin = bytes;
+    for (size_t i = 0; i < nt; ++i) {
+        Off& o = off[i];
+        if (!o.pre) continue;
+        const size_t p = o.p_, q = o.q_, k = std::max(o.k_, 1);
+        o.ux = take(p * k * sizeof(cplx));
+        if (npass >= 3) o.tmp = take(q * k * sizeof(cplx));
+    }
+    const size_t zero_end = bytes;
     const size_t off_k = take(nt * sizeof(int));
     const size_t off_d = take(nt * sizeof(double));
     char* ws = static_cast<char*>(dalloc_bytes(c, bytes));
@@ -622,9 +630,15 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         }
         if (f.empty()) break;
         run_qr_factor(c, f);
+        std::vector<UpperHItem> uh;
+        std::vector<int2> ut;
         for (size_t i = 0; i < nt; ++i)
-            if (off[i].pre)
-                launch_upper_h(qr[i][j].a, qr[i][j].m, off[i].q_, P(off[i].y[j]), c->stream);
+            if (off[i].pre) {
+                const int q = off[i].q_;
+                for (size_t b = 0; b * 256 < (size_t)q * q; ++b) ut.push_back(make_int2((int)uh.size(), (int)b));
+                uh.push_back(UpperHItem{qr[i][j].a, qr[i][j].m, q, P(off[i].y[j])});
+            }
+        launch_upper_h_batch(upload(c, uh), upload(c, ut), (int)ut.size(), c->stream);
     }
 
     std::vector<JacItem> jac(nt);
@@ -663,13 +677,13 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     const bool even = npass % 2 == 0;
     std::vector<int> uq, vq;  // factors to apply on each side, innermost first
     for (int j = npass; j >= 1; --j) (j % 2 ? uq : vq).push_back(j);
+    if (zero_end > zero_begin)
+        launch_fill(reinterpret_cast<double*>(ws + zero_begin), (zero_end - zero_begin) / sizeof(double), 0.0,
+                    c->stream);
     std::vector<GatherItem> pre_g, to_ux;
     for (size_t i = 0; i < nt; ++i) {
         const Off& o = off[i];
         if (!o.pre) continue;
-        const size_t kk = std::max(o.k_, 1);
-        launch_fill(reinterpret_cast<double*>(P(o.ux)), 2 * (size_t)o.p_ * kk, 0.0, c->stream);
-        if (npass >= 3) launch_fill(reinterpret_cast<double*>(P(o.tmp)), 2 * (size_t)o.q_ * kk, 0.0, c->stream);
         GatherItem g{};
         g.src = even ? jac[i].x : jac[i].v; g.ld_src = o.q_; g.rows = o.q_; g.perm = sel[i].perm;
         g.sigma_sorted = sel[i].sigma_sorted;
