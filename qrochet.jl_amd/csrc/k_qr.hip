@@ -147,13 +147,16 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
                 vr = cmul(P(c, r), h.scale);
                 P(c, r) = vr;
             }
+            // all 16 loads issue before their first use (column min(k, nb - 1) is always
+            // inside the panel; the values of k >= nb are discarded)
+            double2 pk[QR_NB];
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) pk[k] = P(min(k, nb - 1), r);
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
-                if (k > c && k < nb) dots[k] = cfmac(vr, P(k, r), dots[k]);
-                else if (k < c) {
-                    const double2 vk = P(k, r);  // r >= c > k: below column k's diagonal
-                    dots[k] = cfmac(vk, vr, dots[k]);
-                }
+                // k < c: v_k^H v (r >= c > k is below column k's diagonal); k > c: v^H a_k
+                const double2 t = (k < c) ? cfmac(pk[k], vr, dots[k]) : cfmac(vr, pk[k], dots[k]);
+                if (k != c && k < nb) dots[k] = t;
             }
         }
         block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
@@ -182,10 +185,13 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
         for (int r = tid; r < rows; r += 256) {
             if (r < c) continue;
             const double2 vr = (r == c) ? make_double2(1.0, 0.0) : P(c, r);
+            double2 pk[QR_NB];
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) pk[k] = P(min(k, nb - 1), r);
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
                 if (k > c && k < nb) {
-                    const double2 x = csub(P(k, r), cmul(w[k], vr));
+                    const double2 x = csub(pk[k], cmul(w[k], vr));
                     P(k, r) = x;
                     if (k == c + 1 && r > c + 1) part += cabs2(x);
                 }
