@@ -17,13 +17,13 @@ namespace qh {
 
 constexpr int QR_TC = 16;  // target columns per apply workgroup
 
-// block-wide sum of K complex values per thread (256 threads); result in out[K]
+// block-wide sum of K complex values per thread (NW waves); result in out[K]
 __device__ __forceinline__ double2 shfl_xor2(double2 a, int off) {
     return make_double2(__shfl_xor(a.x, off, 64), __shfl_xor(a.y, off, 64));
 }
 
-template <int K>
-__device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K */, double2* out /* K */) {
+template <int K, int NW = 4>
+__device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* NW*K */, double2* out /* K */) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if constexpr (K >= 2 && K <= 16 && (K & (K - 1)) == 0) {
         // recursive halving: at the step with offset o the lane keeps the upper or
@@ -66,21 +66,26 @@ __device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* 4*K 
     __syncthreads();
     if (threadIdx.x < K) {
         double2 s = cz();
-        for (int w = 0; w < 4; ++w) s = cadd(s, sRed[w * K + threadIdx.x]);
+        for (int w = 0; w < NW; ++w) s = cadd(s, sRed[w * K + threadIdx.x]);
         out[threadIdx.x] = s;
     }
     __syncthreads();
 }
 
-// sum over the workgroup of one real value per thread, returned to every thread
-// (one barrier; `sRed4` must not be rewritten before the next barrier)
-__device__ __forceinline__ double block_sum_all(double x, double* sRed4) {
+// sum over the workgroup (NW waves) of one real value per thread, returned to every
+// thread (one barrier; `sRedW` must not be rewritten before the next barrier); the
+// wave sums are added as a pairwise tree
+template <int NW = 4>
+__device__ __forceinline__ double block_sum_all(double x, double* sRedW) {
+    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    if (lane == 0) sRed4[wv] = x;
+    if (lane == 0) sRedW[wv] = x;
     __syncthreads();
-    return (sRed4[0] + sRed4[1]) + (sRed4[2] + sRed4[3]);
+    const double lo = (sRedW[0] + sRedW[1]) + (sRedW[2] + sRedW[3]);
+    if constexpr (NW == 4) return lo;
+    else return lo + ((sRedW[4] + sRedW[5]) + (sRedW[6] + sRedW[7]));
 }
 
 // LAPACK zlarfg: H^H (alpha; x) = (beta; 0), H = I - tau v v^H, v = (1; x·scale), beta real
@@ -105,8 +110,8 @@ __device__ __forceinline__ Reflector make_reflector(double2 alpha, double xnorm2
 }
 
 struct PanelShared {
-    double2 sRed[4 * QR_NB];
-    double sRedN[2][4];
+    double2 sRed[8 * QR_NB];  // up to 8 waves
+    double sRedN[2][8];
     double2 sDot[QR_NB];
     double2 sT[QR_NB][QR_NB + 1];
     double2 sScale;
@@ -119,28 +124,29 @@ struct PanelShared {
 // relative to the panel corner), shared by the LDS-resident and the in-place
 // global-memory variants.  Leaves R on and above the diagonal, the scaled
 // reflectors below it, and the compact-WY triangle in sh.sT.
-template <class Ref>
+template <int NT, class Ref>
 __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
+    constexpr int NW = NT / 64;
     const int tid = threadIdx.x;
-    // Thread tid owns rows r = tid + 256 j for the whole panel, so the reflector's
+    // Thread tid owns rows r = tid + NT j for the whole panel, so the reflector's
     // scaling, the dots, the trailing update and the next column's norm touch only
     // the thread's own rows; the only cross-thread data are the reductions and
     // alpha = P(c, c), whose owner writes it before the barrier of the next norm.
     double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
-    for (int r = tid; r < rows; r += 256)
+    for (int r = tid; r < rows; r += NT)
         if (r > 0) part += cabs2(P(0, r));
     for (int c = 0; c < nb; ++c) {
         // every thread forms the reflector from the reduced norm (no serial section);
         // the two norm buffers alternate so a fast thread cannot overwrite partial
         // sums that a slow thread of the previous column still reads
-        const double xnorm2 = block_sum_all(part, sh.sRedN[c & 1]);
+        const double xnorm2 = block_sum_all<NW>(part, sh.sRedN[c & 1]);
         const Reflector h = make_reflector(P(c, c), xnorm2);
         // v below the diagonal (scaled in place) and its dots with the panel:
         // dots[k] = v^H a_k (k > c), dots[k] = v_k^H v (k < c)
         double2 dots[QR_NB];
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
-        for (int r = tid; r < rows; r += 256) {
+        for (int r = tid; r < rows; r += NT) {
             if (r < c) continue;
             double2 vr = make_double2(1.0, 0.0);
             if (r > c) {
@@ -159,7 +165,7 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
                 if (k != c && k < nb) dots[k] = t;
             }
         }
-        block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
+        block_sum<QR_NB, NW>(dots, sh.sRed, sh.sDot);
         // (every thread has read alpha = P(c, c) before the barriers above)
         if (tid == c) P(c, c) = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
@@ -182,7 +188,7 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? cmul(ctau, sh.sDot[k]) : cz();
         part = 0.0;
-        for (int r = tid; r < rows; r += 256) {
+        for (int r = tid; r < rows; r += NT) {
             if (r < c) continue;
             const double2 vr = (r == c) ? make_double2(1.0, 0.0) : P(c, r);
             double2 pk[QR_NB];
@@ -203,16 +209,17 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
 
 // panel factorisation in place in global memory (any number of rows; the panel's
 // 16 columns stay in L2 across the column loop)
+template <int NT>
 __device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
     const int tid = threadIdx.x;
     const int rows = it.m - j0;
     const int nb = min(QR_NB, it.n - j0);
     double2* base = it.a + j0 + (size_t)it.lda * j0;
     const size_t lda = it.lda;
-    qr_panel_columns(rows, nb, sh, [&](int c, int r) -> double2& { return base[r + lda * c]; });
+    qr_panel_columns<NT>(rows, nb, sh, [&](int c, int r) -> double2& { return base[r + lda * c]; });
     __syncthreads();
     // explicit reflectors to v, zeros below the diagonal of R
-    for (int e = tid; e < nb * rows; e += 256) {
+    for (int e = tid; e < nb * rows; e += NT) {
         const int c = e / rows, r = e % rows;
         const size_t vi = (j0 + r) + (size_t)it.m * (j0 + c);
         double2& ar = base[r + lda * c];
@@ -226,27 +233,28 @@ __device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
         }
     }
     double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
-    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+    for (int e = tid; e < QR_NB * QR_NB; e += NT) {
         const int i = e % QR_NB, j = e / QR_NB;
         t[i + QR_NB * j] = (i < nb && j < nb) ? sh.sT[i][j] : cz();
     }
 }
 
-constexpr int QR_REG_ROWS = 512;  // panels of up to this many rows are factored in registers
+// panels of up to 2·NT rows are factored in registers (two rows per thread)
 
 // Panel factorisation with the panel in registers: thread tid owns rows tid and
-// tid + 256 of the (rows × QR_NB) panel, so the dots and the trailing update of a
+// tid + NT of the (rows × QR_NB) panel, so the dots and the trailing update of a
 // column read no memory.  The column index is a template parameter (one
 // instantiation per column), so every register index is static.  Cross-thread
 // data: the norm and dot reductions and alpha = P(c, c), which thread c (owner of
 // row c) publishes before the norm's barrier.  Same arithmetic, in the same
 // order, as qr_panel_columns.
-template <int C>
+template <int C, int NT>
 __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& part, int rows, int nb,
                                               PanelShared& sh) {
+    constexpr int NW = NT / 64;
     const int tid = threadIdx.x;
     if (tid == C) sh.sAlpha[C & 1] = P[0][C];
-    const double xnorm2 = block_sum_all(part, sh.sRedN[C & 1]);
+    const double xnorm2 = block_sum_all<NW>(part, sh.sRedN[C & 1]);
     const Reflector h = make_reflector(sh.sAlpha[C & 1], xnorm2);
     // v on the owned rows (0 on rows above the diagonal and past the panel)
     double2 vr[2];
@@ -255,7 +263,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
     for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int r = tid + 256 * j;
+        const int r = tid + NT * j;
         vr[j] = cz();
         if (r == C) vr[j] = make_double2(1.0, 0.0);
         if (r > C && r < rows) {
@@ -270,7 +278,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
             }
         }
     }
-    block_sum<QR_NB>(dots, sh.sRed, sh.sDot);
+    block_sum<QR_NB, NW>(dots, sh.sRed, sh.sDot);
     if (tid == C) P[0][C] = make_double2(h.beta, 0.0);
     if (tid < QR_NB) {
         const int i = tid;
@@ -294,7 +302,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
     part = 0.0;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int r = tid + 256 * j;
+        const int r = tid + NT * j;
         if (r >= C && r < rows) {
 #pragma unroll
             for (int k = C + 1; k < QR_NB; ++k) {
@@ -307,10 +315,11 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
         }
     }
     if constexpr (C + 1 < QR_NB) {
-        if (C + 1 < nb) qr_reg_column<C + 1>(P, part, rows, nb, sh);
+        if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, part, rows, nb, sh);
     }
 }
 
+template <int NT>
 __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelShared& sh) {
     const int tid = threadIdx.x;
     const int rows = it.m - j0;
@@ -320,22 +329,22 @@ __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelSha
     double2 P[2][QR_NB];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int r = tid + 256 * j;
+        const int r = tid + NT * j;
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) P[j][k] = (r < rows && k < nb) ? a0[r + lda * k] : cz();
     }
     double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int r = tid + 256 * j;
+        const int r = tid + NT * j;
         if (r > 0 && r < rows) part += cabs2(P[j][0]);
     }
-    qr_reg_column<0>(P, part, rows, nb, sh);
+    qr_reg_column<0, NT>(P, part, rows, nb, sh);
     __syncthreads();
     // R (upper part) back to a, the explicit reflectors to v
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int r = tid + 256 * j;
+        const int r = tid + NT * j;
         if (r >= rows) continue;
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) {
@@ -352,18 +361,29 @@ __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelSha
         }
     }
     double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
-    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+    for (int e = tid; e < QR_NB * QR_NB; e += NT) {
         const int i = e % QR_NB, j = e / QR_NB;
         t[i + QR_NB * j] = (i < nb && j < nb) ? sh.sT[i][j] : cz();
     }
 }
 
+// panels of at most 512 rows (the host sends taller batches to k_qr_panel_tall; the
+// global-memory variant is kept out of this kernel and k_qr_step, whose register
+// budget it would raise)
 __global__ __launch_bounds__(256) void k_qr_panel(const QrItem* __restrict__ items, int j0) {
     __shared__ PanelShared sh;
     const QrItem it = items[blockIdx.x];
     if (j0 >= it.n) return;
-    if (it.m - j0 <= QR_REG_ROWS) qr_panel_regs(it, j0, sh);
-    else qr_panel_global(it, j0, sh);
+    qr_panel_regs<256>(it, j0, sh);
+}
+
+// 512 threads: panels of up to 1024 rows in registers, taller ones in global memory
+__global__ __launch_bounds__(512) void k_qr_panel_tall(const QrItem* __restrict__ items, int j0) {
+    __shared__ PanelShared sh;
+    const QrItem it = items[blockIdx.x];
+    if (j0 >= it.n) return;
+    if (it.m - j0 <= 1024) qr_panel_regs<512>(it, j0, sh);
+    else qr_panel_global<512>(it, j0, sh);
 }
 
 // Y[j0:, c0 : c0 + tc] <- (I - V T' V^H) Y[j0:, c0 : c0 + tc],  T' = T^H (adjoint) or T,
@@ -571,7 +591,9 @@ __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__
 // panel j0's block reflector to the trailing tiles beyond that panel.  The two
 // parts touch disjoint columns and disjoint reflector storage, so they run in one
 // launch and the panel's latency hides behind the trailing update.
-__global__ __launch_bounds__(256) void k_qr_step(const QrItem* __restrict__ items, int count,
+// (two waves per SIMD: the trailing-update workgroups need the occupancy)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_qr_step(
+    const QrItem* __restrict__ items, int count,
                                                  const QrApplyItem* __restrict__ app, const int2* __restrict__ tiles,
                                                  int j0) {
     __shared__ union {
@@ -582,8 +604,7 @@ __global__ __launch_bounds__(256) void k_qr_step(const QrItem* __restrict__ item
         const QrItem it = items[blockIdx.x];
         const int jn = j0 + QR_NB;
         if (jn >= it.n) return;
-        if (it.m - jn <= QR_REG_ROWS) qr_panel_regs(it, jn, sh.p);
-        else qr_panel_global(it, jn, sh.p);
+        qr_panel_regs<256>(it, jn, sh.p);  // at most 512 rows (host routing)
         return;
     }
     const int2 tt = tiles[blockIdx.x - count];
@@ -599,6 +620,10 @@ __global__ __launch_bounds__(256) void k_qr_step(const QrItem* __restrict__ item
 void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
     if (count <= 0) return;
     k_qr_panel<<<count, 256, 0, s>>>(d_items, j0);
+}
+void launch_qr_panel_tall(const QrItem* d_items, int count, int j0, hipStream_t s) {
+    if (count <= 0) return;
+    k_qr_panel_tall<<<count, 512, 0, s>>>(d_items, j0);
 }
 void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s) {
     if (ntiles <= 0) return;

@@ -165,13 +165,17 @@ struct QrApplyItem {
     int ncols;
     int trailing;   // 1: columns from j0 + panel width (factorisation update); 0: all columns
 };
-// panel j0 of every item (grid = items; items with n <= j0 return at once)
+// panel j0 of every item (grid = items; items with n <= j0 return at once); every
+// item has at most 512 rows
 void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s);
+// the same for any height, 512-thread workgroups (up to 1024 rows in registers)
+void launch_qr_panel_tall(const QrItem* d_items, int count, int j0, hipStream_t s);
 // apply panel j0's block reflector: adjoint = 1 -> Q_panel^H (factorisation), 0 -> Q_panel
 // tile map: (item, column tile of qr_tile_cols() columns)
 void launch_qr_apply(const QrApplyItem* d_items, const int2* d_tiles, int ntiles, int j0, int adjoint, hipStream_t s);
-// factorisation look-ahead: panel j0 + QR_NB of every item (workgroups [0, count)) and
-// panel j0's trailing update on the tiles (item, k >= 1) beyond it, in one launch
+// factorisation look-ahead: panel j0 + QR_NB of every item (workgroups [0, count),
+// at most 512 rows) and panel j0's trailing update on the tiles (item, k >= 1) beyond
+// it, in one launch
 void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, const int2* d_tiles, int ntiles,
                     int j0, hipStream_t s);
 // y <- Q y with all block reflectors of each item (tile map: item, tile of QR_TC columns)

@@ -506,11 +506,12 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
     // trailing tiles of a panel step: tile 0 is the next panel's columns, tiles >= 1
     // the columns beyond it
     std::vector<int2> next_tiles, rest_tiles;
-    int nmax = 0;
+    int nmax = 0, mmax = 0;
     const int tc = qr_tile_cols();
     double flops = 0;
     for (size_t i = 0; i < jobs.size(); ++i) {
         const QrJob& j = jobs[i];
+        mmax = std::max(mmax, j.m);
         items[i] = QrItem{j.a, j.m, j.n, j.lda, j.v, j.t};
         app[i] = QrApplyItem{j.v, j.t, j.m, j.n, j.a, j.lda, j.n, 1};
         nmax = std::max(nmax, j.n);
@@ -524,6 +525,16 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
     int2* dn = upload(c, next_tiles);
     int2* dr = upload(c, rest_tiles);
     Timed tm(c, "qr", flops, 0.0);
+    if (mmax > 512) {
+        // panels of up to 1024 rows in registers need 512-thread workgroups, which the
+        // fused look-ahead step (256 threads) does not have: panel, then trailing update
+        for (int j0 = 0; j0 < nmax; j0 += QR_NB) {
+            launch_qr_panel_tall(di, (int)items.size(), j0, c->stream);
+            launch_qr_apply(da, dn, (int)next_tiles.size(), j0, 1, c->stream);
+            launch_qr_apply(da, dr, (int)rest_tiles.size(), j0, 1, c->stream);
+        }
+        return;
+    }
     // look-ahead: after panel j0, update only the next panel's columns, then factor
     // that panel in the same launch as the rest of panel j0's trailing update
     launch_qr_panel(di, (int)items.size(), 0, c->stream);
