@@ -379,8 +379,9 @@ def test_batched_layer_with_multichunk_descriptor_arena(qh, monkeypatch):
 
 
 def test_theta_taller_than_lds_panel(qh):
-    """A 640×640 θ (χ = 320): the QR preconditioner factors its panels in place in
-    global memory (panels of more than 512 rows do not fit the LDS variant)."""
+    """A 640×640 θ (χ = 320): the QR preconditioner's panels are taller than 512 rows,
+    so they are factored by the 512-thread register variant (up to 1024 rows) without
+    the look-ahead step."""
     from oracle import mpo as om
 
     rng = np.random.default_rng(23)
@@ -398,6 +399,26 @@ def test_theta_taller_than_lds_panel(qh):
     z = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in (9, 10, 11, 12)]) / qh.norm(psi) ** 2
     zr = om.local_z(ref)[[8, 9, 10, 11]]
     assert np.max(np.abs(z - zr)) < 1e-10
+
+
+def test_theta_taller_than_register_panels(qh):
+    """A 1100×66 θ (bond dimensions 550, 80, 33 on four sites): the first QR panels
+    have more than 1024 rows and are factored in place in global memory by 512-thread
+    workgroups, the later ones in registers.  The evolved state and the new Λ match
+    the oracle."""
+    rng = np.random.default_rng(29)
+    arrays = wl.random_mps_arrays(rng, 4, 2, [550, 80, 33])
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    g = wl.haar_gate4(rng)
+    qh.evolve_(psi, qh.gate(g, 2, 3))
+    oc.evolve(ref, oc.gate2(g, 2))
+    assert psi.bond_dims() == ref.bond_dims() == [550, 66, 33]
+    lg, lr = psi.lambda_(2), ref.lam[1]
+    assert np.max(np.abs(lg - lr)) < 1e-10 * lr[0]
+    vg = oc.statevector(oc.Chain(psi.internal_arrays(), psi.lambdas()))
+    vr = oc.statevector(ref)
+    assert np.max(np.abs(vg - vr)) < 1e-10 * np.max(np.abs(vr))
 
 
 def test_workload2_full_size_time_reversal(qh):
