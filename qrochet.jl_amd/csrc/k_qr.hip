@@ -121,9 +121,9 @@ struct PanelShared {
 };
 
 // Column loop of a panel factorisation on a panel view P(c, r) (column c, row r
-// relative to the panel corner), shared by the LDS-resident and the in-place
-// global-memory variants.  Leaves R on and above the diagonal, the scaled
-// reflectors below it, and the compact-WY triangle in sh.sT.
+// relative to the panel corner) in global memory, NT threads.  Leaves R on and
+// above the diagonal, the scaled reflectors below it, and the compact-WY triangle
+// in sh.sT.
 template <int NT, class Ref>
 __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
     constexpr int NW = NT / 64;
@@ -239,10 +239,8 @@ __device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
     }
 }
 
-// panels of up to 2·NT rows are factored in registers (two rows per thread)
-
-// Panel factorisation with the panel in registers: thread tid owns rows tid and
-// tid + NT of the (rows × QR_NB) panel, so the dots and the trailing update of a
+// Panel factorisation with the panel in registers, for panels of up to 2·NT rows:
+// thread tid owns rows tid and tid + NT of the (rows × QR_NB) panel, so the dots and the trailing update of a
 // column read no memory.  The column index is a template parameter (one
 // instantiation per column), so every register index is static.  Cross-thread
 // data: the norm and dot reductions and alpha = P(c, c), which thread c (owner of
