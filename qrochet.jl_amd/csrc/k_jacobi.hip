@@ -220,6 +220,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     __syncthreads();
     if (!sFlag) return;
 
+    // the eigensolver and the sort are a latency-bound chain; their waves get issue
+    // priority over the co-resident workgroups' streaming Gram and update phases
+    __builtin_amdgcn_s_setprio(2);
+
     // ---------------- EVD of G by cyclic Jacobi ----------------
     // A round applies NP disjoint rotations.  Thread (pb, qb) owns the 2×2 block of G
     // at (row pair pb, column pair qb) and the 2×2 block of Q at rows {2pb, 2pb+1} ×
@@ -354,6 +358,8 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
     }
     __syncthreads();
     }
+
+    __builtin_amdgcn_s_setprio(0);
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
     // Register-blocked: thread (rg, cg) accumulates rows rg + 32a (a < RA) × columns
