@@ -14,6 +14,8 @@
 #include "qh_kernels.h"
 #include "qh_complex.h"
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 namespace qh {
 
 __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int& b) {
@@ -75,7 +77,8 @@ __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j
 }
 
 template <int BC>
-__global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict__ items,
+// three waves per SIMD: the MFMA accumulators fit in the VGPRs without spilling
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_jacobi_round(const JacItem* __restrict__ items,
                                                       const int2* __restrict__ tiles, int round, int sort_sweeps,
                                                       int debug, int max_inner) {
     constexpr int B = BC / 2;
@@ -150,47 +153,95 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacItem* __restrict_
             sX[e / CH][e % CH] = pre[s2];
         }
     };
-    fetch(it.x, p, 0);
-    for (int row0 = 0; row0 < p; row0 += CH) {
-        stash();
-        __syncthreads();
-        if (row0 + CH < p) fetch(it.x, p, row0 + CH);
-#pragma unroll 2
-        for (int rr = wv; rr < CH; rr += 4) {
-            double2 x1i[S], x1j[S], x2i[S], x2j[S];
+    if constexpr (BC == 32) {
+        // On the matrix cores (v_mfma_f64_16x16x4): a 16-column block is one MFMA tile,
+        // and one LDS value per lane is both operands of X_I^H X_J (A[i][k] = x_i at row
+        // k, B[k][j] = x_j at row k); the conjugate is in the sums Re = ar br + ai bi,
+        // Im = ar bi - ai br.  Wave wv takes rows wv + 4t (t < 8) of each 32-row chunk,
+        // four per step; result register q of lane l is G[l / 16 + 4 q][l % 16].
+        f64x4 g11r = {0.0, 0.0, 0.0, 0.0}, g11i = g11r, g12r = g11r, g12i = g11r, g22r = g11r, g22i = g11r;
+        fetch(it.x, p, 0);
+        for (int row0 = 0; row0 < p; row0 += CH) {
+            stash();
+            __syncthreads();
+            if (row0 + CH < p) fetch(it.x, p, row0 + CH);
 #pragma unroll
-            for (int a = 0; a < S; ++a) {
-                x1i[a] = sX[ti * S + a][rr];
-                x1j[a] = sX[tj * S + a][rr];
-                x2i[a] = sX[B + ti * S + a][rr];
-                x2j[a] = sX[B + tj * S + a][rr];
+            for (int s4 = 0; s4 < CH / 16; ++s4) {
+                const int rr = wv + 4 * (4 * s4 + (lane >> 4));
+                const double2 x1 = sX[lane & 15][rr], x2 = sX[B + (lane & 15)][rr];
+                g11r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.x, g11r, 0, 0, 0);
+                g11r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x1.y, g11r, 0, 0, 0);
+                g11i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.y, g11i, 0, 0, 0);
+                g11i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, -x1.x, g11i, 0, 0, 0);
+                g12r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x2.x, g12r, 0, 0, 0);
+                g12r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x2.y, g12r, 0, 0, 0);
+                g12i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x2.y, g12i, 0, 0, 0);
+                g12i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, -x2.x, g12i, 0, 0, 0);
+                g22r = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.x, x2.x, g22r, 0, 0, 0);
+                g22r = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, x2.y, g22r, 0, 0, 0);
+                g22i = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.x, x2.y, g22i, 0, 0, 0);
+                g22i = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, -x2.x, g22i, 0, 0, 0);
             }
-#pragma unroll
-            for (int a = 0; a < S; ++a)
-#pragma unroll
-                for (int b = 0; b < S; ++b) {
-                    acc11[a][b] = cfmac(x1i[a], x1j[b], acc11[a][b]);
-                    acc12[a][b] = cfmac(x1i[a], x2j[b], acc12[a][b]);
-                    acc22[a][b] = cfmac(x2i[a], x2j[b], acc22[a][b]);
-                }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    for (int w = 0; w < 4; ++w) {
-        if (wv == w) {
+        for (int w = 0; w < 4; ++w) {
+            if (wv == w) {
 #pragma unroll
-            for (int a = 0; a < S; ++a)
-#pragma unroll
-                for (int b = 0; b < S; ++b) {
-                    double2& g11 = sG[ti * S + a][tj * S + b];
-                    double2& g12 = sG[ti * S + a][B + tj * S + b];
-                    double2& g22 = sG[B + ti * S + a][B + tj * S + b];
-                    g11 = (w == 0) ? acc11[a][b] : cadd(g11, acc11[a][b]);
-                    g12 = (w == 0) ? acc12[a][b] : cadd(g12, acc12[a][b]);
-                    g22 = (w == 0) ? acc22[a][b] : cadd(g22, acc22[a][b]);
+                for (int q = 0; q < 4; ++q) {
+                    const int r = (lane >> 4) + 4 * q, c = lane & 15;
+                    const double2 a11 = make_double2(g11r[q], g11i[q]);
+                    const double2 a12 = make_double2(g12r[q], g12i[q]);
+                    const double2 a22 = make_double2(g22r[q], g22i[q]);
+                    sG[r][c] = (w == 0) ? a11 : cadd(sG[r][c], a11);
+                    sG[r][B + c] = (w == 0) ? a12 : cadd(sG[r][B + c], a12);
+                    sG[B + r][B + c] = (w == 0) ? a22 : cadd(sG[B + r][B + c], a22);
                 }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    } else {
+        fetch(it.x, p, 0);
+        for (int row0 = 0; row0 < p; row0 += CH) {
+            stash();
+            __syncthreads();
+            if (row0 + CH < p) fetch(it.x, p, row0 + CH);
+    #pragma unroll 2
+            for (int rr = wv; rr < CH; rr += 4) {
+                double2 x1i[S], x1j[S], x2i[S], x2j[S];
+    #pragma unroll
+                for (int a = 0; a < S; ++a) {
+                    x1i[a] = sX[ti * S + a][rr];
+                    x1j[a] = sX[tj * S + a][rr];
+                    x2i[a] = sX[B + ti * S + a][rr];
+                    x2j[a] = sX[B + tj * S + a][rr];
+                }
+    #pragma unroll
+                for (int a = 0; a < S; ++a)
+    #pragma unroll
+                    for (int b = 0; b < S; ++b) {
+                        acc11[a][b] = cfmac(x1i[a], x1j[b], acc11[a][b]);
+                        acc12[a][b] = cfmac(x1i[a], x2j[b], acc12[a][b]);
+                        acc22[a][b] = cfmac(x2i[a], x2j[b], acc22[a][b]);
+                    }
+            }
+            __syncthreads();
+        }
+        for (int w = 0; w < 4; ++w) {
+            if (wv == w) {
+    #pragma unroll
+                for (int a = 0; a < S; ++a)
+    #pragma unroll
+                    for (int b = 0; b < S; ++b) {
+                        double2& g11 = sG[ti * S + a][tj * S + b];
+                        double2& g12 = sG[ti * S + a][B + tj * S + b];
+                        double2& g22 = sG[B + ti * S + a][B + tj * S + b];
+                        g11 = (w == 0) ? acc11[a][b] : cadd(g11, acc11[a][b]);
+                        g12 = (w == 0) ? acc12[a][b] : cadd(g12, acc12[a][b]);
+                        g22 = (w == 0) ? acc22[a][b] : cadd(g22, acc22[a][b]);
+                    }
+            }
+            __syncthreads();
+        }
     }
 
     // make G exactly Hermitian so that every test below sees the same value for (i, j)
