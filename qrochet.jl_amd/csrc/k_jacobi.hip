@@ -445,6 +445,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         ufetch(base, rows, 0, 0);
 #pragma unroll 1
         for (int row0 = 0; row0 < rows; row0 += RB) {
+          if constexpr (BC == 32) {
+            // On the matrix cores (v_mfma_f64_16x16x4): wave mw owns panel rows
+            // [32 mw, 32 mw + 32) as two 16-row tiles, times Q's two 16-column tiles; a
+            // complex product is four real ones (re·re − im·im, re·im + im·re).  Operand
+            // lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l
+            // is D[l / 16 + 4 q][l % 16].
+            const int ml = tid & 63, mw = tid >> 6;
+            f64x4 accR[2][2], accI[2][2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accR[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    accI[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+                }
+#pragma unroll 1
+            for (int i0 = 0; i0 < BC; i0 += IC) {
+                ustash();
+                __syncthreads();
+                if (i0 + IC < BC) ufetch(base, rows, row0, i0 + IC);
+                else if (row0 + RB < rows) ufetch(base, rows, row0 + RB, 0);
+#pragma unroll
+                for (int kk = 0; kk < IC; kk += 4) {
+                    double2 xa[2], qb[2];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) xa[a] = sU[kk + (ml >> 4)][32 * mw + 16 * a + (ml & 15)];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) qb[b] = sQ[i0 + kk + (ml >> 4)][16 * b + (ml & 15)];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) {
+                            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accR[a][b], 0, 0, 0);
+                            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, -qb[b].y, accR[a][b], 0, 0, 0);
+                            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].y, accI[a][b], 0, 0, 0);
+                            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].x, accI[a][b], 0, 0, 0);
+                        }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int gc = sCol[16 * b + (ml & 15)];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int grow = row0 + 32 * mw + 16 * a + (ml >> 4) + 4 * q;
+                        if (gc >= 0 && grow < rows)
+                            base[grow + (size_t)rows * gc] = make_double2(accR[a][b][q], accI[a][b][q]);
+                    }
+                }
+          } else {
             double2 o[RA][JG];
 #pragma unroll
             for (int a = 0; a < RA; ++a)
@@ -479,6 +532,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if (gc >= 0 && grow < rows) base[grow + (size_t)rows * gc] = o[a][b];
                 }
             }
+          }
         }
     }
 }
