@@ -62,17 +62,20 @@ __device__ __forceinline__ JRot jacobi_rot(double gaa, double gbb, double2 gab, 
     return r;
 }
 // (x_a, x_b) <- (x_a, x_b) J
+// (with se = s e folded in: one product and two FMAs per real output)
 __device__ __forceinline__ void rot_cols(double2& xa, double2& xb, const JRot& j) {
-    const double2 ev = make_double2(j.ex, j.ey);
-    const double2 na = csub(cscale(xa, j.c), cscale(cmul(cconj(ev), xb), j.s));
-    xb = cadd(cscale(cmul(ev, xa), j.s), cscale(xb, j.c));
+    const double sx = j.s * j.ex, sy = j.s * j.ey;  // x_a' = c x_a - conj(se) x_b ; x_b' = se x_a + c x_b
+    const double2 na = make_double2(fma(-sy, xb.y, fma(-sx, xb.x, j.c * xa.x)),
+                                    fma(sy, xb.x, fma(-sx, xb.y, j.c * xa.y)));
+    xb = make_double2(fma(-sy, xa.y, fma(sx, xa.x, j.c * xb.x)), fma(sy, xa.x, fma(sx, xa.y, j.c * xb.y)));
     xa = na;
 }
 // (y_a; y_b) <- J^H (y_a; y_b)
 __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j) {
-    const double2 ev = make_double2(j.ex, j.ey);
-    const double2 na = csub(cscale(ya, j.c), cscale(cmul(ev, yb), j.s));
-    yb = cadd(cscale(cmul(cconj(ev), ya), j.s), cscale(yb, j.c));
+    const double sx = j.s * j.ex, sy = j.s * j.ey;  // y_a' = c y_a - se y_b ; y_b' = conj(se) y_a + c y_b
+    const double2 na = make_double2(fma(sy, yb.y, fma(-sx, yb.x, j.c * ya.x)),
+                                    fma(-sy, yb.x, fma(-sx, yb.y, j.c * ya.y)));
+    yb = make_double2(fma(sy, ya.y, fma(sx, ya.x, j.c * yb.x)), fma(-sy, ya.x, fma(sx, ya.y, j.c * yb.y)));
     ya = na;
 }
 
