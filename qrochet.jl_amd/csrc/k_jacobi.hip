@@ -23,6 +23,16 @@ __device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int&
     if (k == 0) { a = round; b = m; }
     else { a = (round + k) % m; b = (round - k + m) % m; }
 }
+// rr_pair(round + 1, k, nidx) from rr_pair(round, k, nidx)
+__device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
+    const int m = nidx - 1;
+    if (k == 0) {
+        ++a;
+    } else {
+        a = (a + 1 == m) ? 0 : a + 1;
+        b = (b + 1 == m) ? 0 : b + 1;
+    }
+}
 
 // Rotation of the Hermitian 2×2 problem [[g_aa, g_ab], [conj(g_ab), g_bb]]:
 // J = [[c, s e], [-s conj(e), c]] with e = g_ab/|g_ab| and t = s/c the smaller
@@ -305,12 +315,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int sweep = 0; sweep < inner_cap; ++sweep) {
         if (tid == 0) sFlag = 0;
         __syncthreads();
+        // the round's pairs, advanced from round to round without the modulo
+        int ap, bp, aq, bq;
+        rr_pair(0, pb, BC, ap, bp);
+        rr_pair(0, qb, BC, aq, bq);
 #pragma unroll 1
         for (int rnd = 0; rnd < BC - 1; ++rnd) {
             if (owner) {
-                int ap, bp, aq, bq;
-                rr_pair(rnd, pb, BC, ap, bp);
-                rr_pair(rnd, qb, BC, aq, bq);
                 // each lane solves its column pair; the row pair's rotation comes from
                 // the lane of the same wave that owns diagonal block (pb, pb)
                 const JRot jq = jacobi_rot(gA[aq][aq].x, gA[bq][bq].x, gA[aq][bq], tol2, gfloor);
@@ -346,6 +357,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 gB[ap][aq] = g00; gB[ap][bq] = g01; gB[bp][aq] = g10; gB[bp][bq] = g11;
                 if (pb == qb && jp.s != 0.0) sFlag = 1;
             }
+            rr_next(pb, BC, ap, bp);
+            rr_next(qb, BC, aq, bq);
             __syncthreads();
             double2 (*gt)[LP] = gA;
             gA = gB;
