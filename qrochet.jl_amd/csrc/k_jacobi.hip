@@ -41,11 +41,14 @@ __device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
 // pair is already orthogonal, |g_ab|^2 <= tol2·g_aa·g_bb, or a column is below
 // the deflation floor.
 // With d = g_bb - g_aa and h = sqrt(d^2 + 4|g_ab|^2) (= 2|g_ab| sqrt(1 + zeta^2)):
-// |t| = 2|g_ab| / (|d| + h) and c^2 = 1/(1 + t^2) = (|d| + h) / (2h), so the two
-// square roots are independent and the dependent chain is sqrt, divide, sqrt.
+// |t| = 2|g_ab| / (|d| + h) and c^2 = 1/(1 + t^2) = (|d| + h) / (2h).  The rotation
+// is applied through c and se = s e = sign(d) (2c / (|d| + h)) g_ab, and the new
+// diagonal through t|g_ab| = sign(d) 2|g_ab|^2 / (|d| + h), so neither |g_ab| nor e
+// is formed: two square roots and two divisions, chain sqrt, divide, sqrt.
 struct JRot {
-    double c = 1.0, s = 0.0, ex = 1.0, ey = 0.0, naa = 0.0, nbb = 0.0;
+    double c = 1.0, sx = 0.0, sy = 0.0, naa = 0.0, nbb = 0.0;  // se = (sx, sy)
 };
+__device__ __forceinline__ bool rotates(const JRot& j) { return j.sx != 0.0 || j.sy != 0.0; }
 __device__ __forceinline__ bool needs_rotation(double gaa, double gbb, double2 gab, double tol2, double gfloor) {
     return gaa > gfloor && gbb > gfloor && cabs2(gab) > tol2 * gaa * gbb;
 }
@@ -56,25 +59,24 @@ __device__ __forceinline__ JRot jacobi_rot(double gaa, double gbb, double2 gab, 
     if (needs_rotation(gaa, gbb, gab, tol2, gfloor)) {
         const double g2 = cabs2(gab);
         const double d = gbb - gaa;
-        const double mag = sqrt(g2);
         const double h = sqrt(fma(d, d, 4.0 * g2));
         const double den = fabs(d) + h;
-        const double tabs = (2.0 * mag) / den;
-        const double t = d >= 0.0 ? tabs : -tabs;
+        const double rden = 1.0 / den;
         r.c = sqrt(den / (2.0 * h));
-        r.s = r.c * t;
-        const double rmag = 1.0 / mag;
-        r.ex = gab.x * rmag;
-        r.ey = gab.y * rmag;
-        r.naa = gaa - t * mag;
-        r.nbb = gbb + t * mag;
+        const double sg = d >= 0.0 ? 2.0 : -2.0;
+        const double f = sg * r.c * rden;
+        r.sx = f * gab.x;
+        r.sy = f * gab.y;
+        const double tm = sg * g2 * rden;
+        r.naa = gaa - tm;
+        r.nbb = gbb + tm;
     }
     return r;
 }
 // (x_a, x_b) <- (x_a, x_b) J
-// (with se = s e folded in: one product and two FMAs per real output)
+// (one product and two FMAs per real output)
 __device__ __forceinline__ void rot_cols(double2& xa, double2& xb, const JRot& j) {
-    const double sx = j.s * j.ex, sy = j.s * j.ey;  // x_a' = c x_a - conj(se) x_b ; x_b' = se x_a + c x_b
+    const double sx = j.sx, sy = j.sy;  // x_a' = c x_a - conj(se) x_b ; x_b' = se x_a + c x_b
     const double2 na = make_double2(fma(-sy, xb.y, fma(-sx, xb.x, j.c * xa.x)),
                                     fma(sy, xb.x, fma(-sx, xb.y, j.c * xa.y)));
     xb = make_double2(fma(-sy, xa.y, fma(sx, xa.x, j.c * xb.x)), fma(sy, xa.x, fma(sx, xa.y, j.c * xb.y)));
@@ -82,7 +84,7 @@ __device__ __forceinline__ void rot_cols(double2& xa, double2& xb, const JRot& j
 }
 // (y_a; y_b) <- J^H (y_a; y_b)
 __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j) {
-    const double sx = j.s * j.ex, sy = j.s * j.ey;  // y_a' = c y_a - se y_b ; y_b' = conj(se) y_a + c y_b
+    const double sx = j.sx, sy = j.sy;  // y_a' = c y_a - se y_b ; y_b' = conj(se) y_a + c y_b
     const double2 na = make_double2(fma(sy, yb.y, fma(-sx, yb.x, j.c * ya.x)),
                                     fma(-sy, yb.x, fma(-sx, yb.y, j.c * ya.y)));
     yb = make_double2(fma(sy, ya.y, fma(sx, ya.x, j.c * yb.x)), fma(-sy, ya.x, fma(sx, ya.y, j.c * yb.y)));
@@ -329,12 +331,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const int src = (pb % (64 / NP)) * NP + pb;
                 JRot jp;
                 jp.c = __shfl(jq.c, src, 64);
-                jp.s = __shfl(jq.s, src, 64);
-                jp.ex = __shfl(jq.ex, src, 64);
-                jp.ey = __shfl(jq.ey, src, 64);
+                jp.sx = __shfl(jq.sx, src, 64);
+                jp.sy = __shfl(jq.sy, src, 64);
                 if (pb == qb) jp = jq;
                 double2 g00 = gA[ap][aq], g01 = gA[ap][bq], g10 = gA[bp][aq], g11 = gA[bp][bq];
-                if (jq.s != 0.0) {  // columns:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
+                if (rotates(jq)) {  // columns:  x_a' = c x_a - s conj(e) x_b ;  x_b' = s e x_a + c x_b
                     double2 q00 = sQ[2 * pb][aq], q01 = sQ[2 * pb][bq];
                     double2 q10 = sQ[2 * pb + 1][aq], q11 = sQ[2 * pb + 1][bq];
                     rot_cols(g00, g01, jq);
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     rot_cols(q10, q11, jq);
                     sQ[2 * pb][aq] = q00; sQ[2 * pb][bq] = q01; sQ[2 * pb + 1][aq] = q10; sQ[2 * pb + 1][bq] = q11;
                 }
-                if (jp.s != 0.0) {
+                if (rotates(jp)) {
                     if (pb == qb) {  // the diagonal block becomes diag(λ_a, λ_b)
                         g00 = make_double2(jp.naa, 0.0);
                         g11 = make_double2(jp.nbb, 0.0);
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                 }
                 gB[ap][aq] = g00; gB[ap][bq] = g01; gB[bp][aq] = g10; gB[bp][bq] = g11;
-                if (pb == qb && jp.s != 0.0) sFlag = 1;
+                if (pb == qb && rotates(jp)) sFlag = 1;
             }
             rr_next(pb, BC, ap, bp);
             rr_next(qb, BC, aq, bq);
