@@ -178,6 +178,38 @@ def test_batched_layer_equals_single_calls(qh):
     assert np.max(np.abs(za - zb)) < 1e-12
 
 
+def test_batched_layers_on_saturated_random_states(qh):
+    """Two batched Haar brickwork layers on random MPS already at chimax = 256, as in the
+    batched uneven workload: θ is 512 x 512 and full rank from the first layer, so the
+    SVDs start from the least favourable case (no small spectrum to converge first).
+    Kept bond dimensions and the Λ spectra of the updated bonds match the oracle."""
+    rng = np.random.default_rng(21)
+    grng = np.random.default_rng(22)
+    n, chi = 20, 256
+    arrays = [_random_chain(rng, n, 2, chi) for _ in range(2)]
+    gpus = [qh.from_internal(a) for a in arrays]
+    refs = [oc.Chain([x.copy() for x in a]) for a in arrays]
+    for layer in range(2):
+        ups = []
+        for psi, ref in zip(gpus, refs):
+            for b in wl.brickwork_bonds(n, layer):
+                g = wl.haar_gate4(grng)
+                ups.append((psi, qh.gate(g, b, b + 1)))
+                oc.evolve(ref, oc.gate2(g, b), maxdim=chi)
+        qh.evolve_layer_(ups, maxdim=chi)
+    for psi, ref in zip(gpus, refs):
+        assert psi.bond_dims() == ref.bond_dims()
+        assert max(ref.bond_dims()) == chi
+        checked = 0
+        for lg, lr in zip(psi.lambdas(), ref.lam):
+            if lr is None:
+                continue
+            assert lg is not None
+            assert np.max(np.abs(lg - lr)) < TOL
+            checked += 1
+        assert checked >= n // 2
+
+
 @pytest.mark.parametrize("batched", [False, True])
 def test_ising_trotter_prefix_parity(qh, batched):
     """Workload 2 (n=64, cutoff 1e-12, maxdim 256) for its first 2 Trotter steps."""
