@@ -468,14 +468,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // complex product is four real ones (re·re − im·im, re·im + im·re).  Operand
             // lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l
             // is D[l / 16 + 4 q][l % 16].
+            // Three real products per complex tile: P1 = Xr Qr, P2 = Xi Qi,
+            // P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.
             const int ml = tid & 63, mw = tid >> 6;
-            f64x4 accR[2][2], accI[2][2];
+            f64x4 accP[2][2], accQ[2][2], accS[2][2];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accR[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-                    accI[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    accP[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    accQ[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    accS[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
                 }
 #pragma unroll 1
             for (int i0 = 0; i0 < BC; i0 += IC) {
@@ -490,14 +493,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     for (int a = 0; a < 2; ++a) xa[a] = sU[kk + (ml >> 4)][32 * mw + 16 * a + (ml & 15)];
 #pragma unroll
                     for (int b = 0; b < 2; ++b) qb[b] = sQ[i0 + kk + (ml >> 4)][16 * b + (ml & 15)];
+                    double xs[2], qs[2];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) xs[a] = xa[a].x + xa[a].y;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) qs[b] = qb[b].x + qb[b].y;
 #pragma unroll
                     for (int a = 0; a < 2; ++a)
 #pragma unroll
                         for (int b = 0; b < 2; ++b) {
-                            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accR[a][b], 0, 0, 0);
-                            accR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, -qb[b].y, accR[a][b], 0, 0, 0);
-                            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].y, accI[a][b], 0, 0, 0);
-                            accI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].x, accI[a][b], 0, 0, 0);
+                            accP[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accP[a][b], 0, 0, 0);
+                            accQ[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].y, accQ[a][b], 0, 0, 0);
+                            accS[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[a], qs[b], accS[a][b], 0, 0, 0);
                         }
                 }
                 __syncthreads();
@@ -511,7 +518,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     for (int q = 0; q < 4; ++q) {
                         const int grow = row0 + 32 * mw + 16 * a + (ml >> 4) + 4 * q;
                         if (gc >= 0 && grow < rows)
-                            base[grow + (size_t)rows * gc] = make_double2(accR[a][b][q], accI[a][b][q]);
+                            base[grow + (size_t)rows * gc] =
+                                make_double2(accP[a][b][q] - accQ[a][b][q], accS[a][b][q] - accP[a][b][q] - accQ[a][b][q]);
                     }
                 }
           } else {
