@@ -170,11 +170,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     };
     if constexpr (BC == 32) {
         // On the matrix cores (v_mfma_f64_16x16x4): a 16-column block is one MFMA tile,
-        // and one LDS value per lane is both operands of X_I^H X_J (A[i][k] = x_i at row
-        // k, B[k][j] = x_j at row k); the conjugate is in the sums Re = ar br + ai bi,
-        // Im = ar bi - ai br.  Wave wv takes rows wv + 4t (t < 8) of each 32-row chunk,
-        // four per step; result register q of lane l is G[l / 16 + 4 q][l % 16].
-        f64x4 g11r = {0.0, 0.0, 0.0, 0.0}, g11i = g11r, g12r = g11r, g12i = g11r, g22r = g11r, g22i = g11r;
+        // and one LDS value per lane gives the operands of X_I^H X_J (A[i][k] = x_i at
+        // row k, B[k][j] = x_j at row k).  Three real products per complex block:
+        // P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi); Re = P1 + P2, Im = P3 - P1 + P2
+        // (the conjugate of A is in the signs).  Wave wv takes rows wv + 4t (t < 8) of
+        // each 32-row chunk, four per step; result register q of lane l is
+        // G[l / 16 + 4 q][l % 16].
+        f64x4 p11 = {0.0, 0.0, 0.0, 0.0}, q11 = p11, s11 = p11, p12 = p11, q12 = p11, s12 = p11;
+        f64x4 p22 = p11, q22 = p11, s22 = p11;
         fetch(it.x, p, 0);
         for (int row0 = 0; row0 < p; row0 += CH) {
             stash();
@@ -184,18 +187,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             for (int s4 = 0; s4 < CH / 16; ++s4) {
                 const int rr = wv + 4 * (4 * s4 + (lane >> 4));
                 const double2 x1 = sX[lane & 15][rr], x2 = sX[B + (lane & 15)][rr];
-                g11r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.x, g11r, 0, 0, 0);
-                g11r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x1.y, g11r, 0, 0, 0);
-                g11i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.y, g11i, 0, 0, 0);
-                g11i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, -x1.x, g11i, 0, 0, 0);
-                g12r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x2.x, g12r, 0, 0, 0);
-                g12r = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x2.y, g12r, 0, 0, 0);
-                g12i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x2.y, g12i, 0, 0, 0);
-                g12i = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, -x2.x, g12i, 0, 0, 0);
-                g22r = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.x, x2.x, g22r, 0, 0, 0);
-                g22r = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, x2.y, g22r, 0, 0, 0);
-                g22i = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.x, x2.y, g22i, 0, 0, 0);
-                g22i = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, -x2.x, g22i, 0, 0, 0);
+                const double d1 = x1.x - x1.y, e1 = x1.x + x1.y, e2 = x2.x + x2.y, d2 = x2.x - x2.y;
+                p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.x, p11, 0, 0, 0);
+                q11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x1.y, q11, 0, 0, 0);
+                s11 = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, e1, s11, 0, 0, 0);
+                p12 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x2.x, p12, 0, 0, 0);
+                q12 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x2.y, q12, 0, 0, 0);
+                s12 = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, e2, s12, 0, 0, 0);
+                p22 = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.x, x2.x, p22, 0, 0, 0);
+                q22 = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, x2.y, q22, 0, 0, 0);
+                s22 = __builtin_amdgcn_mfma_f64_16x16x4f64(d2, e2, s22, 0, 0, 0);
             }
             __syncthreads();
         }
@@ -204,9 +205,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int r = (lane >> 4) + 4 * q, c = lane & 15;
-                    const double2 a11 = make_double2(g11r[q], g11i[q]);
-                    const double2 a12 = make_double2(g12r[q], g12i[q]);
-                    const double2 a22 = make_double2(g22r[q], g22i[q]);
+                    const double2 a11 = make_double2(p11[q] + q11[q], s11[q] - p11[q] + q11[q]);
+                    const double2 a12 = make_double2(p12[q] + q12[q], s12[q] - p12[q] + q12[q]);
+                    const double2 a22 = make_double2(p22[q] + q22[q], s22[q] - p22[q] + q22[q]);
                     sG[r][c] = (w == 0) ? a11 : cadd(sG[r][c], a11);
                     sG[r][B + c] = (w == 0) ? a12 : cadd(sG[r][B + c], a12);
                     sG[B + r][B + c] = (w == 0) ? a22 : cadd(sG[B + r][B + c], a22);
