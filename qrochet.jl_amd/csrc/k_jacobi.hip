@@ -465,12 +465,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int row0 = 0; row0 < rows; row0 += RB) {
           if constexpr (BC == 32) {
             // On the matrix cores (v_mfma_f64_16x16x4): wave mw owns panel rows
-            // [32 mw, 32 mw + 32) as two 16-row tiles, times Q's two 16-column tiles; a
-            // complex product is four real ones (re·re − im·im, re·im + im·re).  Operand
-            // lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l
-            // is D[l / 16 + 4 q][l % 16].
-            // Three real products per complex tile: P1 = Xr Qr, P2 = Xi Qi,
-            // P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.
+            // [32 mw, 32 mw + 32) as two 16-row tiles, times Q's two 16-column tiles.  A
+            // complex tile product is three real ones: P1 = Xr Qr, P2 = Xi Qi,
+            // P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.  Operand lanes:
+            // A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l is
+            // D[l / 16 + 4 q][l % 16].
             const int ml = tid & 63, mw = tid >> 6;
             f64x4 accP[2][2], accQ[2][2], accS[2][2];
 #pragma unroll
