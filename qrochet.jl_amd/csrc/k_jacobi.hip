@@ -101,8 +101,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     constexpr int CH = 32;
     constexpr int S = B / 8;     // Gram sub-tile per thread and block (8 × 8 threads per wave)
     constexpr int NP = BC / 2;   // rotations per inner round
-    // the update stages 128-row × 8-column panels of X through the same LDS as the
-    // Gram's 32-row chunks
+    // the vector-ALU update (BC = 16) stages 128-row × 8-column panels of X through the
+    // same LDS as the Gram's 32-row chunks
     constexpr int RB = 128, IC = 8;
     constexpr int SX_ELEMS = (BC * (CH + 1) > IC * (RB + 1)) ? BC * (CH + 1) : IC * (RB + 1);
     __shared__ double2 sG[BC][LP];
@@ -431,10 +431,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __builtin_amdgcn_s_setprio(0);
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
-    // Register-blocked: thread (rg, cg) accumulates rows rg + 32a (a < RA) × columns
-    // cg·JG + b (b < JG) of a 128-row panel, so every LDS value feeds RA or JG complex
-    // FMAs.  The panel's X is staged IC inner columns at a time (prefetched into
-    // registers one stage ahead); the sum runs over i = 0..BC-1 in order.
+    // 128-row panels.  BC = 32 runs on the matrix cores with its operands loaded
+    // directly (below).  Otherwise register-blocked on the vector ALUs: thread (rg, cg)
+    // accumulates rows rg + 32a (a < RA) × columns cg·JG + b (b < JG) of a panel, so
+    // every LDS value feeds RA or JG complex FMAs; the panel's X is staged IC inner
+    // columns at a time (prefetched into registers one stage ahead); the sum runs over
+    // i = 0..BC-1 in order.
     constexpr int JG = BC / 8;      // output columns per thread
     constexpr int RA = RB / 32;     // output rows per thread
     constexpr int UPT = IC * RB / 256;
@@ -460,7 +462,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int mat = 0; mat < 2; ++mat) {
         double2* base = mat ? it.v : it.x;
         const int rows = mat ? it.q : p;
-        ufetch(base, rows, 0, 0);
+        if constexpr (BC != 32) ufetch(base, rows, 0, 0);
 #pragma unroll 1
         for (int row0 = 0; row0 < rows; row0 += RB) {
           if constexpr (BC == 32) {
@@ -480,34 +482,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     accQ[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
                     accS[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
                 }
-#pragma unroll 1
-            for (int i0 = 0; i0 < BC; i0 += IC) {
-                ustash();
-                __syncthreads();
-                if (i0 + IC < BC) ufetch(base, rows, row0, i0 + IC);
-                else if (row0 + RB < rows) ufetch(base, rows, row0 + RB, 0);
+            // A operands straight from HBM: the wave reads only its own rows (16
+            // consecutive rows of one column per 16 lanes), so the update needs no
+            // staging and no barrier; Q is read-only in LDS after the eigensolver.
+            const int r0w = row0 + 32 * mw + (ml & 15);
 #pragma unroll
-                for (int kk = 0; kk < IC; kk += 4) {
-                    double2 xa[2], qb[2];
+            for (int st = 0; st < BC / 4; ++st) {
+                const int c = sCol[4 * st + (ml >> 4)];
+                double2 xa[2], qb[2];
 #pragma unroll
-                    for (int a = 0; a < 2; ++a) xa[a] = sU[kk + (ml >> 4)][32 * mw + 16 * a + (ml & 15)];
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) qb[b] = sQ[i0 + kk + (ml >> 4)][16 * b + (ml & 15)];
-                    double xs[2], qs[2];
-#pragma unroll
-                    for (int a = 0; a < 2; ++a) xs[a] = xa[a].x + xa[a].y;
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) qs[b] = qb[b].x + qb[b].y;
-#pragma unroll
-                    for (int a = 0; a < 2; ++a)
-#pragma unroll
-                        for (int b = 0; b < 2; ++b) {
-                            accP[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accP[a][b], 0, 0, 0);
-                            accQ[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].y, accQ[a][b], 0, 0, 0);
-                            accS[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[a], qs[b], accS[a][b], 0, 0, 0);
-                        }
+                for (int a = 0; a < 2; ++a) {
+                    const int r = r0w + 16 * a;
+                    xa[a] = (c >= 0 && r < rows) ? base[r + (size_t)rows * c] : cz();
                 }
-                __syncthreads();
+#pragma unroll
+                for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
+                double xs[2], qs[2];
+#pragma unroll
+                for (int a = 0; a < 2; ++a) xs[a] = xa[a].x + xa[a].y;
+#pragma unroll
+                for (int b = 0; b < 2; ++b) qs[b] = qb[b].x + qb[b].y;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        accP[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accP[a][b], 0, 0, 0);
+                        accQ[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].y, accQ[a][b], 0, 0, 0);
+                        accS[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[a], qs[b], accS[a][b], 0, 0, 0);
+                    }
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
