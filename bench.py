@@ -45,14 +45,41 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_workload(qh, arrays, layers, trunc, precision="fp64", stop_layer=None):
+def update_flops(d, chil, chim, chir):
+    """Algorithmic work of one two-site update (θ build + gate, thin SVD) in real
+    flops: F_gemm = 8 d² χl χm χr + 8 d⁴ χl χr (complex MAC = 8 flops) and the
+    Golub–Van Loan R-SVD count ×4 for complex, F_svd = 4 (14 m k² + 8 k³) with
+    m = max(d χl, d χr), k = min(d χl, d χr)."""
+    f_gemm = 8.0 * d * d * chil * chim * chir + 8.0 * d ** 4 * chil * chir
+    m, k = max(d * chil, d * chir), min(d * chil, d * chir)
+    return f_gemm, 4.0 * (14.0 * m * k * k + 8.0 * k ** 3)
+
+
+def update_bytes(d, chil, chim, chir, kept):
+    """Algorithmic HBM bytes of one FP64 update: read both site tensors and the
+    gate, write the two new site tensors and the new Λ."""
+    return 16.0 * (d * chil * chim + d * chim * chir + d ** 4) + 16.0 * (d * chil * kept + kept * d * chir) + 8.0 * kept
+
+
+def run_workload(qh, arrays, layers, trunc, precision="fp64", stop_layer=None, account=None):
+    """Runs the layers on a fresh upload of `arrays`.  `account`, if a dict, receives
+    the algorithmic flops and bytes of every update from the bond dimensions
+    before and after each layer (host metadata only: no device synchronisation)."""
     psi = qh.from_internal(arrays)
     nup = 0
     for li, layer in enumerate(layers):
         if stop_layer is not None and li >= stop_layer:
             break
         ups = [(psi, qh.gate(g, b, b + 1)) for b, g in layer]
-        qh.evolve_layer_(ups, threshold=trunc["threshold"], maxdim=trunc["maxdim"], precision=precision)
+        before = psi.bond_dims() if account is not None else None
+        kept, _ = qh.evolve_layer_(ups, threshold=trunc["threshold"], maxdim=trunc["maxdim"], precision=precision)
+        if account is not None:
+            dims = [1] + before + [1]
+            for (b, _), k in zip(layer, kept):
+                fg, fs = update_flops(2, dims[b - 1], dims[b], dims[b + 1])
+                account["f_gemm"] = account.get("f_gemm", 0.0) + fg
+                account["f_svd"] = account.get("f_svd", 0.0) + fs
+                account["bytes"] = account.get("bytes", 0.0) + update_bytes(2, dims[b - 1], dims[b], dims[b + 1], k)
         nup += len(ups)
     psi.ctx.synchronize()
     return psi, nup
@@ -211,6 +238,78 @@ def mpo_leg(qh, wl, n, chi, D, sweeps, cpu, cpu_chi):
     return out
 
 
+def contractions_leg(qh, wl, psi, reps=3):
+    """`expect` / `overlap` / `norm` / `canonize!` on a saturated workload state
+    (n = 64, chi = 256): all <Z_i> plus every <Z_i Z_j> with |i - j| <= 4 from one
+    `qh_expect_many`, the overlap with a copy, the norm, and `canonize!` of a copy
+    (the Vidal form: a right-to-left QR sweep and a left-to-right SVD sweep).
+    Median wall time over `reps` calls (each ends with its result on the host)."""
+    ctx = psi.ctx
+    n = psi.n
+    terms = [[qh.gate(wl.Z, i)] for i in range(1, n + 1)]
+    terms += [[qh.gate(wl.Z, i), qh.gate(wl.Z, j)] for i in range(1, n) for j in range(i + 1, min(n, i + 4) + 1)]
+    other = psi.copy()
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return round(float(np.median(ts)) * 1e3, 3)
+
+    out = {"state": f"workload state after 10 Trotter steps (n={n}, max bond {max(psi.bond_dims())})",
+           "expect_many_terms": len(terms),
+           "expect_many_ms": timed(lambda: qh.expect_many(psi, terms)),
+           "overlap_ms": timed(lambda: qh.overlap(psi, other)),
+           "norm_ms": timed(lambda: qh.norm(psi))}
+    copies = [psi.copy() for _ in range(reps)]
+    it = iter(copies)
+    out["canonize_ms"] = timed(lambda: qh.canonize_(next(it)))
+    return out
+
+
+def cpu_baseline_leg(sat_arrays, sat_lams, layer, trunc, n_updates, reps):
+    """The CPU oracle (numpy/scipy LAPACK zgesdd restatement of the update) on the
+    first `n_updates` updates of one saturated layer, repeated `reps` times from the
+    same state with every host thread and then with one thread; medians."""
+    from oracle import chain as oc
+
+    try:
+        from threadpoolctl import threadpool_info, threadpool_limits
+    except ImportError:  # pragma: no cover - threadpoolctl ships with scikit-learn
+        threadpool_info = threadpool_limits = None
+    ups = layer[:n_updates]
+
+    def one_rep():
+        ref = oc.Chain([a.copy() for a in sat_arrays], [None if l is None else l.copy() for l in sat_lams])
+        c0 = time.perf_counter()
+        for b, g in ups:
+            oc.evolve(ref, oc.gate2(g, b), threshold=trunc["threshold"], maxdim=trunc["maxdim"])
+        return len(ups) / (time.perf_counter() - c0)
+
+    threads = None
+    if threadpool_info is not None:
+        threads = max((i.get("num_threads") or 0) for i in threadpool_info()) or None
+    one_rep()  # warm-up (LAPACK workspace queries, page faults)
+    rates = [one_rep() for _ in range(reps)]
+    rates1 = []
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1):
+            one_rep()
+            rates1 = [one_rep() for _ in range(reps)]
+    return {"value": round(float(np.median(rates)), 3), "unit": "updates/s", "cores": threads or os.cpu_count(),
+            "kind": "port",
+            "value_1_thread": round(float(np.median(rates1)), 3) if rates1 else None,
+            "reps": reps, "spread": [round(min(rates), 3), round(max(rates), 3)],
+            "spread_1_thread": [round(min(rates1), 3), round(max(rates1), 3)] if rates1 else None,
+            "sample": f"the first {len(ups)} updates of a saturated (chi=256, theta 512x512) layer of the same "
+                      f"workload from the same state, numpy/scipy-openblas zgesdd restatement of the update, "
+                      f"median of {reps} repetitions after one warm-up, all host threads and 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,7 +317,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--trotter", type=int, default=20, help="Trotter steps per bench step")
     ap.add_argument("--sample-layers", type=int, default=3, help="saturated layers for plain/CPU samples")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-updates", type=int, default=8, help="updates per CPU-baseline repetition")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="CPU-baseline repetitions (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serving", choices=("sample", "full", "off"), default="sample")
@@ -294,9 +394,12 @@ def main():
     n_updates = sum(len(l) for l in layers)
     ctx = qh.default_context()
 
+    # algorithmic flops / bytes of one step (every step does the same updates: the
+    # inputs and the arithmetic are identical, so are the kept dimensions)
+    account = {}
     for w in range(args.warmup):
         tw = time.perf_counter()
-        psi, _ = run_workload(qh, arrays, layers, trunc)
+        psi, _ = run_workload(qh, arrays, layers, trunc, account=account if w == 0 else None)
         log(f"warmup {w}: {time.perf_counter() - tw:.2f} s, max bond {max(psi.bond_dims())}")
     ctx.synchronize()
     ctx.reset_timing()
@@ -304,7 +407,7 @@ def main():
     ctx.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        psi, nup = run_workload(qh, arrays, layers, trunc)
+        psi, nup = run_workload(qh, arrays, layers, trunc, account=account if not account else None)
         log(f"timed step {k} done at {time.perf_counter() - t0:.2f} s")
     ctx.synchronize()
     t1 = time.perf_counter()
@@ -315,19 +418,22 @@ def main():
     value = n_updates * args.steps / elapsed
     final_dims = psi.bond_dims()
 
+    # Roofline of the dominant kernel, the Jacobi round.  `achieved` follows the
+    # algorithmic count of a two-site update (F_gemm + F_svd at each update's own
+    # (χl, χm, χr), summed over the timed steps) divided by the summed duration of
+    # the Jacobi launches (HIP events on the library's stream), against the FP64
+    # peak: the fraction of the FP64 roof the update path would reach if all of
+    # its algorithmic work ran in the time of this one kernel.
     jr = timers["jacobi_round"]
-    avg_ms = jr["ms"] / max(jr["launches"], 1)
-    achieved_tf = jr["flops"] / (jr["ms"] * 1e-3) / 1e12 if jr["ms"] > 0 else 0.0
-    achieved_gbs = jr["bytes"] / (jr["ms"] * 1e-3) / 1e9 if jr["ms"] > 0 else 0.0
-    frac_c = achieved_tf / FP64_PEAK_TFLOPS
-    frac_m = achieved_gbs / HBM_PEAK_GBS
-    if frac_c >= frac_m:
-        roof = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(frac_c, 4)}
-    else:
-        roof = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(frac_m, 4)}
-    # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+    jr_s = jr["ms"] * 1e-3
+    f_alg = (account["f_gemm"] + account["f_svd"]) * args.steps
+    b_alg = account["bytes"] * args.steps
+    launches = max(jr["launches"], 1)
+    achieved_tf = f_alg / jr_s / 1e12 if jr_s > 0 else 0.0
+    roof = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4)}
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated,
+    # and counting L2 misses served by the Infinity Cache as well as HBM reads)
     roof["traffic"] = None
     tf = os.path.join(ROOT, "profiles", "k_jacobi_round_traffic.json")
     if os.path.exists(tf):
@@ -336,11 +442,22 @@ def main():
         roof["traffic"] = t["traffic_bytes_per_launch"]
         roof["traffic_source"] = f"profiles/k_jacobi_round_traffic.json ({t['workload']})"
     roof["kernel"] = "k_jacobi_round<32> (block one-sided Jacobi round, FP64)"
-    roof["avg_launch_ms"] = round(avg_ms, 5)
+    roof["avg_launch_ms"] = round(jr["ms"] / launches, 5)
     roof["launches"] = jr["launches"]
-    roof["flops_per_launch"] = jr["flops"] / max(jr["launches"], 1)
-    roof["bytes_per_launch"] = jr["bytes"] / max(jr["launches"], 1)
-    roof["other_bound_frac"] = round(min(frac_c, frac_m), 4)
+    roof["algorithmic_flops_per_launch"] = f_alg / launches
+    roof["algorithmic_bytes_per_launch"] = b_alg / launches
+    # what the kernel itself streams and computes (counted on the device: Gram reads of
+    # every checked block pair, X/V reads and writes of every rotated one)
+    roof["kernel_streamed_bytes_per_launch"] = jr["bytes"] / launches
+    roof["kernel_streamed_over_algorithmic_bytes"] = round(jr["bytes"] / b_alg, 1) if b_alg > 0 else None
+    roof["kernel_executed_flops_per_launch"] = jr["flops"] / launches
+    roof["kernel_executed_tflops"] = round(jr["flops"] / jr_s / 1e12, 3) if jr_s > 0 else None
+    # end to end: roofline time of all updates of the timed steps over the wall time
+    t_star = account["f_gemm"] * args.steps / (FP64_PEAK_TFLOPS * 1e12) + \
+        account["f_svd"] * args.steps / (FP64_PEAK_TFLOPS * 1e12)
+    end_to_end = {"t_star_s": round(t_star, 4), "t_measured_s": round(elapsed, 4),
+                  "frac": round(t_star / elapsed, 4),
+                  "algorithmic_gflop_per_step": round((account["f_gemm"] + account["f_svd"]) / 1e9, 1)}
 
     # saturated sample: layers of Trotter steps 11.. (χ = 256 in the bulk)
     sat_start = max(0, min(30, len(layers) - args.sample_layers))
@@ -380,6 +497,8 @@ def main():
                        "sum_bond_dims": int(sum(psi_m.bond_dims())),
                        "sum_bond_dims_fp64": int(sum(psi_b.bond_dims()))}
     extra["mixed_precision_sample"] = mixed
+    extra["contractions"] = contractions_leg(qh, wl, psi_b)
+    log("contraction sample done")
     if not args.no_plain:
         psi_p = state_sat.copy()
         ctx.synchronize()
@@ -396,31 +515,7 @@ def main():
     log("plain sample done")
     cpu = None
     if not args.no_cpu:
-        from oracle import chain as oc
-
-        threads = None
-        try:
-            from threadpoolctl import threadpool_info
-
-            threads = max((i.get("num_threads") or 0) for i in threadpool_info()) or None
-        except Exception:
-            threads = None
-        ref = oc.Chain([a.copy() for a in sat_arrays], [None if l is None else l.copy() for l in sat_lams])
-        done = 0
-        c0 = time.perf_counter()
-        for layer in sample:
-            for b, g in layer:
-                oc.evolve(ref, oc.gate2(g, b), threshold=trunc["threshold"], maxdim=trunc["maxdim"])
-                done += 1
-                if time.perf_counter() - c0 > args.cpu_seconds:
-                    break
-            if time.perf_counter() - c0 > args.cpu_seconds:
-                break
-        ct = time.perf_counter() - c0
-        cpu = {"value": round(done / ct, 3), "unit": "updates/s", "cores": threads or os.cpu_count(),
-               "kind": "port",
-               "sample": f"{done} saturated (chi=256) updates of the same workload, numpy/LAPACK zgesdd oracle, "
-                         f"{ct:.1f} s"}
+        cpu = cpu_baseline_leg(sat_arrays, sat_lams, sample[0], trunc, args.cpu_updates, args.cpu_reps)
 
     out = {
         "metric": "2-site gate+SVD updates/s at chi=256 (MPS TEBD, Ising Trotter workload)",
@@ -442,6 +537,7 @@ def main():
             "final_max_bond": max(final_dims),
         },
         "roofline": roof,
+        "end_to_end_roofline": end_to_end,
         "cpu_baseline": cpu,
         "kernel_ms": {k: round(v["ms"], 3) for k, v in timers.items()},
         "jacobi_sweeps": timers["jacobi_sweep"]["launches"],

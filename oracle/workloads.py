@@ -73,6 +73,32 @@ def brickwork_bonds(n, layer):
     return list(range(start, n, 2))
 
 
+def isometric_environment_state(rng, n=8, bond=4, chi=8):
+    """Internal arrays and Λ's of a state whose environment is isometric on both
+    sides of `bond` (sites < bond left-isometric including the Λ on bond-1, sites
+    > bond+1 right-isometric including the Λ on bond+1), with non-trivial Λ
+    vectors on bond-1, bond and bond+1."""
+    arrays = random_mps_arrays(rng, n, 2, max_profile(n, 2, chi))
+    for i in range(bond - 1):  # left-canonize sites 1 .. bond-1
+        cl, d, cr = arrays[i].shape
+        q, r = np.linalg.qr(arrays[i].reshape(cl * d, cr))
+        arrays[i] = q.reshape(cl, d, q.shape[1])
+        arrays[i + 1] = np.tensordot(r, arrays[i + 1], axes=(1, 0))
+    for i in range(n - 1, bond, -1):  # right-canonize sites n .. bond+2
+        cl, d, cr = arrays[i].shape
+        q, r = np.linalg.qr(arrays[i].reshape(cl, d * cr).T)
+        arrays[i] = q.T.reshape(q.shape[1], d, cr)
+        arrays[i - 1] = np.tensordot(arrays[i - 1], r.T, axes=(2, 0))
+    lams = [None] * (n - 1)
+    lo = rng.random(arrays[bond - 2].shape[2]) + 0.2
+    hi = rng.random(arrays[bond].shape[2]) + 0.2
+    arrays[bond - 2] = arrays[bond - 2] / lo[None, None, :]   # A Λ stays isometric
+    arrays[bond + 1] = arrays[bond + 1] / hi[:, None, None]
+    lams[bond - 2], lams[bond] = lo, hi
+    lams[bond - 1] = rng.random(arrays[bond - 1].shape[2]) + 0.2
+    return arrays, lams
+
+
 # ---------------------------------------------------------------------------
 # workload 1: n=16, χ=32, 8 Haar brickwork layers, maxdim=32
 # ---------------------------------------------------------------------------

@@ -752,6 +752,89 @@ void launch_select_items(const SelectItem* d_items, const int2* d_norm_tiles, in
 }
 
 // ---------------------------------------------------------------------------
+// orthonormal completion of kept columns below the deflation floor
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double splitmix_unit(unsigned long long k) {
+    k += 0x9e3779b97f4a7c15ull;
+    k = (k ^ (k >> 30)) * 0xbf58476d1ce4e5b9ull;
+    k = (k ^ (k >> 27)) * 0x94d049bb133111ebull;
+    k ^= k >> 31;
+    return (double)(k >> 11) * 0x1.0p-52 - 1.0;  // uniform in [-1, 1)
+}
+
+constexpr int COMPLETE_MAXQ = SELECT_MAXQ;
+__global__ __launch_bounds__(256) void k_complete(const CompleteItem* __restrict__ items) {
+    __shared__ double2 sDot[COMPLETE_MAXQ];
+    __shared__ double sRed[4];
+    __shared__ int sFirst;
+    const CompleteItem it = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int kept = *it.kept;
+    const double floor2 = it.floor_rel2 * (*it.fro2);
+    if (tid == 0) {
+        int f = kept;
+        while (f > 0 && it.sigma_sorted[f - 1] * it.sigma_sorted[f - 1] <= floor2) --f;
+        sFirst = f;
+    }
+    __syncthreads();
+    const int first = sFirst;
+    if (first >= kept) return;
+    const int p = it.p;
+    // sorted column i is x[:, perm[i]] / σ_i (or x[:, perm[i]] when σ_i = 0)
+    auto col = [&](int i) { return it.x + (size_t)p * it.perm[i]; };
+    auto inv = [&](int i) {
+        const double s = it.sigma_sorted[i];
+        return s > 0.0 ? 1.0 / s : 1.0;
+    };
+    for (int j = first; j < kept; ++j) {
+        double2* z = col(j);
+        // a deterministic pseudo-random start (the same for a matrix alone or in a batch)
+        for (int r = tid; r < p; r += 256) {
+            const unsigned long long key = ((unsigned long long)j << 32) | (unsigned)r;
+            z[r] = make_double2(splitmix_unit(2 * key), splitmix_unit(2 * key + 1));
+        }
+        __syncthreads();
+        // two passes of classical Gram-Schmidt against every column before it
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = tid; i < j; i += 256) {
+                const double2* ci = col(i);
+                double2 acc = cz();
+                for (int r = 0; r < p; ++r) acc = cfmac(ci[r], z[r], acc);
+                sDot[i] = cscale(acc, inv(i));
+            }
+            __syncthreads();
+            for (int r = tid; r < p; r += 256) {
+                double2 v = z[r];
+                for (int i = 0; i < j; ++i) v = csub(v, cmul(sDot[i], cscale(col(i)[r], inv(i))));
+                z[r] = v;
+            }
+            __syncthreads();
+        }
+        double s = 0.0;
+        for (int r = tid; r < p; r += 256) s += cabs2(z[r]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((tid & 63) == 0) sRed[tid >> 6] = s;
+        __syncthreads();
+        const double nrm = sqrt((sRed[0] + sRed[1]) + (sRed[2] + sRed[3]));
+        const double sj = it.sigma_sorted[j];
+        const bool tiny = !(sj >= 1e-280);
+        const double f = (tiny ? 1.0 : sj) / nrm;
+        for (int r = tid; r < p; r += 256) z[r] = cscale(z[r], f);
+        __syncthreads();
+        if (tid == 0 && tiny) {
+            it.sigma_sorted[j] = 0.0;
+            it.lam_out[j] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_complete(const CompleteItem* d_items, int count, hipStream_t s) {
+    if (count > 0) k_complete<<<count, 256, 0, s>>>(d_items);
+}
+
+// ---------------------------------------------------------------------------
 // write-back of kept columns (32 × 32 tiles: j = output column, c = row of src)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ items, const int2* __restrict__ tiles) {
@@ -774,7 +857,7 @@ __global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ i
             double f = 1.0;
             if (g.divide_sigma) {
                 const double sg = g.sigma_sorted[j];
-                f = sg > 0.0 ? 1.0 / sg : 0.0;
+                f = sg > 0.0 ? 1.0 / sg : 1.0;  // σ = 0: the column holds its unit vector
             }
             if (g.rowscale) f *= g.rowscale[(c / g.rowscale_div) % g.rowscale_mod];
             v = cscale(v, f);

@@ -5,8 +5,12 @@
 //   * BF16X3: each FP32 operand split as hi + lo (bfloat16 each) and
 //             x·y ≈ xh·yh + xh·yl + xl·yh on v_mfma_f32_16x16x16_bf16 (the
 //             "bf16x3" product of Henry, Tang & Heinecke, ARITH 2019),
-// with FP32 accumulation.  The 4×4 gate mix, the Vidal scales and the output θ
-// are FP64; the SVD that follows is FP64 throughout.
+// with FP32 accumulation inside each chunk of KC = 16 values of m and the chunk
+// sums gathered in FP64 (for d <= 3).  With one FP32 accumulator over all χm, the
+// accumulated rounding of the sum, not the rounding of the operands, set the error:
+// on saturated χ = 256 TEBD layers the FP32 form's ⟨Z_i⟩ moved by ~2e-5 from FP64,
+// against ~1e-7 for FP32 operands summed exactly.  The 4×4 gate mix, the Vidal
+// scales and the output θ are FP64; the SVD that follows is FP64 throughout.
 //
 // Complex products use four real MFMAs: Re += ar·br + (-ai)·bi, Im += ar·bi + ai·br.
 // A workgroup (4 waves) covers 32 values of l × 32 of r × all (o1, o2); wave w
@@ -56,12 +60,18 @@ __global__ __launch_bounds__(256) void k_theta_mfma(const ThetaItem* __restrict_
     if (tid < D * D * D * D) sG[tid] = it.gate[tid];
 
     f32x4 accR[D][D], accI[D][D];
+    // chunk sums gathered in FP64 (d <= 3; at d = 4 the 128 extra registers would spill)
+    constexpr bool ACC64 = D <= 3;
+    double sumR[ACC64 ? D : 1][ACC64 ? D : 1][4], sumI[ACC64 ? D : 1][ACC64 ? D : 1][4];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int b = 0; b < D; ++b) {
             accR[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
             accI[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if constexpr (ACC64)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) sumR[a][b][t] = sumI[a][b][t] = 0.0;
         }
 
     for (int m0 = 0; m0 < chim; m0 += KC) {
@@ -142,6 +152,20 @@ __global__ __launch_bounds__(256) void k_theta_mfma(const ThetaItem* __restrict_
                     accI[a][b] = i;
                 }
         }
+        if constexpr (ACC64) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        sumR[a][b][t] += (double)accR[a][b][t];
+                        sumI[a][b][t] += (double)accI[a][b][t];
+                    }
+                    accR[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    accI[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+        }
         __syncthreads();
     }
 
@@ -164,8 +188,12 @@ __global__ __launch_bounds__(256) void k_theta_mfma(const ThetaItem* __restrict_
                 for (int o1 = 0; o1 < D; ++o1)
 #pragma unroll
                     for (int o2 = 0; o2 < D; ++o2)
-                        v = cfma(sG[a + D * b + D * D * (o1 + D * o2)],
-                                 make_double2((double)accR[o1][o2][t], (double)accI[o1][o2][t]), v);
+                    {
+                        const double2 acc = ACC64 ? make_double2(sumR[ACC64 ? o1 : 0][ACC64 ? o2 : 0][t],
+                                                                 sumI[ACC64 ? o1 : 0][ACC64 ? o2 : 0][t])
+                                                  : make_double2((double)accR[o1][o2][t], (double)accI[o1][o2][t]);
+                        v = cfma(sG[a + D * b + D * D * (o1 + D * o2)], acc, v);
+                    }
                 v = cscale(v, sc);
                 const size_t row = (size_t)l + (size_t)chil * a;
                 const size_t col = (size_t)b + (size_t)D * r;
@@ -202,7 +230,8 @@ void launch_theta_mfma(const ThetaItem* d_items, const int2* d_tiles, int ntiles
 // Generic batched complex GEMM on MFMA (same items and tiles as k_gemm_f64):
 // C = alpha · op(A) diag(dk) op(B) (+ beta C), rows × rowscale, columns × colscale.
 // A 64×64 tile per workgroup; wave w computes the 32×32 quarter (w % 2, w / 2)
-// as 2×2 MFMA blocks of 16×16.  Operands are staged through LDS in FP32.
+// as 2×2 MFMA blocks of 16×16.  Operands are staged through LDS in FP32; each
+// k-chunk is accumulated in FP32 on the matrix cores and the chunk sums in FP64.
 // ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ items,
@@ -219,12 +248,15 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ 
     const int li = lane & 15, kq = lane >> 4;
 
     f32x4 accR[2][2], accI[2][2];
+    double sumR[2][2][4], sumI[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             accR[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
             accI[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sumR[a][b][t] = sumI[a][b][t] = 0.0;
         }
 
     for (int k0 = 0; k0 < g.k; k0 += TK) {
@@ -311,6 +343,20 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ 
                     accI[a][b] = i;
                 }
         }
+        {  // chunk sums gathered in FP64
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        sumR[a][b][t] += (double)accR[a][b][t];
+                        sumI[a][b][t] += (double)accI[a][b][t];
+                    }
+                    accR[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    accI[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+        }
         __syncthreads();
     }
     const bool use_beta = (g.beta.x != 0.0 || g.beta.y != 0.0);
@@ -325,7 +371,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ 
             for (int t = 0; t < 4; ++t) {
                 const int r = m0 + wm + 16 * a + 4 * kq + t;
                 if (r >= g.m) continue;
-                double2 v = cmul(g.alpha, make_double2((double)accR[a][b][t], (double)accI[a][b][t]));
+                double2 v = cmul(g.alpha, make_double2(sumR[a][b][t], sumI[a][b][t]));
                 v = cscale(v, csc * (g.rowscale ? g.rowscale[r] : 1.0));
                 double2* dst = g.c + r + (size_t)g.ldc * c;
                 if (use_beta) v = cfma(g.beta, *dst, v);

@@ -124,8 +124,27 @@ constexpr int SELECT_MAX_Q = 4096;
 void launch_select_items(const SelectItem* d_items, const int2* d_norm_tiles, int n_norm_tiles, int count,
                          hipStream_t s);
 
+// Kept columns at or below the deflation floor (sorted positions first..kept-1 with
+// σ_j² <= floor_rel2·||X||_F², the Jacobi never rotated them) carry no direction of
+// their own: their singular vectors are replaced by an orthonormal completion of
+// the columns above the floor, written back into the Jacobi matrix as σ_j z_j
+// (z_j a unit vector) so that the gathers' division by σ_j yields z_j; a σ_j below
+// 1e-280 is reported as 0 and z_j stored unscaled.  One workgroup per item; items
+// without such columns return at once.
+struct CompleteItem {
+    cplx* x;                 // Jacobi matrix (p × q, ld p): sorted column j is column perm[j]
+    int p;
+    const int* perm;
+    double* sigma_sorted;
+    double* lam_out;         // kept Λ as written by select (reset to 0 with σ_j)
+    const int* kept;
+    const double* fro2;      // ||X||_F² before the sweeps
+    double floor_rel2;
+};
+void launch_complete(const CompleteItem* d_items, int count, hipStream_t s);
+
 // out[r, j] = src[r, perm[j]] * colfac(j) * rowscale[(r / div) % mod] for j < kept
-//   colfac = 1/sigma_sorted[j] if divide_sigma (0 if sigma == 0) else 1
+//   colfac = 1/sigma_sorted[j] if divide_sigma (1 if sigma == 0: see CompleteItem) else 1
 //   conj_transpose: out is (kept × rows), ld kept, holding the conjugate
 struct GatherItem {
     const cplx* src;

@@ -316,7 +316,7 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
 // ---------------------------------------------------------------------------
 // Jacobi SVD
 // ---------------------------------------------------------------------------
-void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
+void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out) {
     if (items.empty()) return;
     const int block = c->jac_block;
     const int bc = 2 * block;
@@ -324,7 +324,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
     int rmax = 1;
     int* counters = dalloc<int>(c, 2 * items.size() + 2);
     QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 2) * sizeof(int), c->stream));
-    double* fro2 = dalloc<double>(c, items.size());
+    double* fro2 = fro2_out ? fro2_out : dalloc<double>(c, items.size());
+    auto free_fro2 = [&] {
+        if (!fro2_out) dfree(c, fro2);
+    };
     std::vector<int2> frob_tiles;
     std::vector<int> frob_first(1, 0);
     for (size_t i = 0; i < items.size(); ++i) {
@@ -419,7 +422,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
             for (size_t i = 0; i < items.size(); ++i)
                 if (h[2 * i + 1] < 0) msg += " item " + std::to_string(i);
             dfree(c, counters);
-            dfree(c, fro2);
+            free_fro2();
             dfree(c, frob_part);
             throw Error(QH_ERR_NUMERIC, msg);
         }
@@ -454,12 +457,12 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items) {
                 ++shown;
             }
         dfree(c, counters);
-        dfree(c, fro2);
+        free_fro2();
         dfree(c, frob_part);
         throw Error(QH_ERR_NOT_CONVERGED, msg);
     }
     dfree(c, counters);
-    dfree(c, fro2);
+    free_fro2();
     dfree(c, frob_part);
 }
 
@@ -622,6 +625,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     const size_t zero_end = bytes;
     const size_t off_k = take(nt * sizeof(int));
     const size_t off_d = take(nt * sizeof(double));
+    const size_t off_f = take(nt * sizeof(double));
     char* ws = static_cast<char*>(dalloc_bytes(c, bytes));
     auto P = [&](size_t o) { return reinterpret_cast<cplx*>(ws + o); };
     int* d_kept = reinterpret_cast<int*>(ws + off_k);
@@ -677,8 +681,18 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         s.truncate = t.truncate;
         s.renormalize = t.renormalize;
     }
-    run_jacobi(c, jac);
+    double* d_fro2 = reinterpret_cast<double*>(ws + off_f);
+    run_jacobi(c, jac, d_fro2);
     run_select(c, sel);
+    // kept columns the Jacobi left frozen (at or below its deflation floor) get an
+    // orthonormal completion, as LAPACK's factors are orthonormal however small σ is
+    {
+        std::vector<CompleteItem> comp(nt);
+        for (size_t i = 0; i < nt; ++i)
+            comp[i] = CompleteItem{jac[i].x, jac[i].p, sel[i].perm, sel[i].sigma_sorted, sel[i].lam_out,
+                                   sel[i].kept, d_fro2 + i, jac[i].floor_rel2};
+        launch_complete(upload(c, comp), (int)nt, c->stream);
+    }
 
     // ---- preconditioned tasks.  The Jacobi gives Y V_J = W with columns of norm σ;
     // with Wn = W / σ and R_{j-1} = R_j^H Q_j^H unrolled:

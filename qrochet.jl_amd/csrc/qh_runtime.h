@@ -142,8 +142,9 @@ struct Timed {
 // ---- batched building blocks ----
 void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_PREC_FP64);
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);
-// runs set-identity + sweeps until every item converged; throws on sweep limit
-void run_jacobi(qh_context* c, std::vector<JacItem>& items);
+// runs set-identity + sweeps until every item converged; throws on sweep limit.
+// fro2_out (optional, one double per item) receives ||X||_F² and stays valid after.
+void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out = nullptr);
 void run_select(qh_context* c, std::vector<SelectItem>& items);
 void run_gather(qh_context* c, std::vector<GatherItem>& items);
 

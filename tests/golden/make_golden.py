@@ -51,6 +51,96 @@ def config1_outputs():
     )
 
 
+class _SpectrumRecorder:
+    """Records the full spectrum of every SVD the oracle's `evolve` takes (to report
+    how close each threshold decision is to the cut-off)."""
+
+    def __enter__(self):
+        self.last = None
+        self._orig = oc._svd
+
+        def rec(m):
+            u, s, vh = self._orig(m)
+            self.last = s
+            return u, s, vh
+
+        oc._svd = rec
+        return self
+
+    def __exit__(self, *exc):
+        oc._svd = self._orig
+
+
+CONFIG2_STEPS = 6
+CONFIG2_LAMBDA_BONDS = (16, 32, 48)
+
+
+def config2_saturation_outputs(steps=CONFIG2_STEPS):
+    """Workload 2 (n=64, Ising Trotter, cutoff 1e-12, maxdim 256) through `steps`
+    Strang steps: the bond dimension reaches 256 at layer 14 (of 3·steps), so the
+    cut-off decides the kept dimensions while the state grows and the cap decides
+    them once it has saturated.  Records, per update in schedule order, the kept
+    dimension and -- where the cut-off (not the cap) decided it -- the distance of
+    the decisive singular values from the cut-off, |log(σ/1e-12)| of the smallest
+    kept and the largest dropped one; then Λ of three bulk bonds, all unnormalised
+    ⟨Z_i⟩ (`expect`), ⟨ψ|ψ⟩ and the overlap with the initial product state."""
+    arrays, layers, tr, _ = wl.config2(steps=steps)
+    psi = oc.Chain([a.copy() for a in arrays])
+    psi0 = psi.copy()
+    kept, margin = [], []
+    with _SpectrumRecorder() as rec:
+        for layer in layers:
+            for b, g in layer:
+                oc.evolve(psi, oc.gate2(g, b), **tr)
+                k = psi.bond_dims()[b - 1]
+                s = rec.last
+                kept.append(k)
+                if k < min(len(s), tr["maxdim"]):
+                    margin.append(min(abs(np.log(s[k - 1] / tr["threshold"])), abs(np.log(s[k] / tr["threshold"]))))
+                else:
+                    margin.append(np.inf)
+    n = psi.n
+    out = dict(
+        steps=np.int64(steps),
+        kept=np.array(kept, dtype=np.int64),
+        threshold_margin=np.array(margin),
+        lambda_bonds=np.array(CONFIG2_LAMBDA_BONDS, dtype=np.int64),
+        z=np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]),
+        norm2=np.complex128(oc.overlap(psi, psi)),
+        overlap=np.complex128(oc.overlap(psi, psi0)),
+        final_dims=np.array(psi.bond_dims(), dtype=np.int64),
+    )
+    for b in CONFIG2_LAMBDA_BONDS:
+        out[f"lambda{b}"] = psi.lam[b - 1]
+    return out
+
+
+CONFIG3_REDUCED = dict(seed=33, count=16, depth=4, nmin=8, nmax=20)
+
+
+def config3_reduced_outputs():
+    """A reduced batched uneven workload (the shape of workload 3 at a size the
+    oracle finishes in seconds): 16 MPS, n ~ U{8..20}, chimax drawn from
+    {16, 32, 64, 128, 256}, left-canonized random states at the full bond profile,
+    4 Haar brickwork layers, maxdim = chimax (threshold default).  Records every
+    update's kept dimension (MPS by MPS, layer by layer), each MPS's unnormalised
+    ⟨Z_i⟩ and its overlap with its initial state."""
+    items = wl.config3(**CONFIG3_REDUCED)
+    out = dict(count=np.int64(len(items)))
+    for m, it in enumerate(items):
+        psi = oc.Chain([a.copy() for a in it["arrays"]])
+        psi0 = psi.copy()
+        kept = []
+        for layer in it["layers"]:
+            for b, g in layer:
+                oc.evolve(psi, oc.gate2(g, b), maxdim=it["chi"])
+                kept.append(psi.bond_dims()[b - 1])
+        out[f"kept{m}"] = np.array(kept, dtype=np.int64)
+        out[f"z{m}"] = np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, psi.n + 1)])
+        out[f"overlap{m}"] = np.complex128(oc.overlap(psi, psi0))
+    return out
+
+
 def small_exact():
     rng = np.random.default_rng(99)
     n = 6
@@ -75,7 +165,9 @@ def small_exact():
 def main():
     np.savez(path("config1"), **config1_outputs())
     np.savez(path("small_exact"), **small_exact())
-    print("wrote", path("config1"), path("small_exact"))
+    np.savez(path("config2_saturation"), **config2_saturation_outputs())
+    np.savez(path("config3_reduced"), **config3_reduced_outputs())
+    print("wrote", *(path(k) for k in ("config1", "small_exact", "config2_saturation", "config3_reduced")))
 
 
 if __name__ == "__main__":

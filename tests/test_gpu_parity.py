@@ -380,6 +380,72 @@ def test_rank_deficient_theta_without_truncation(qh):
     oc.evolve(ref, oc.gate2(g, 2))
     assert psi.bond_dims() == ref.bond_dims()
     assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-12
+    assert qh.isleftcanonical(psi, 2) and qh.isrightcanonical(psi, 3)
+
+
+def _padded_product_state(rng, n, chi):
+    """A product state embedded in bonds of dimension `chi`: only the (0, :, 0)
+    slice of each site tensor is non-zero, so every θ has rank <= d^2 and the
+    untruncated SVD keeps exactly-zero singular values."""
+    dims = [1] + [chi] * (n - 1) + [1]
+    arrays = []
+    for i in range(n):
+        a = np.zeros((dims[i], 2, dims[i + 1]), dtype=np.complex128)
+        a[0, :, 0] = rng.standard_normal(2) + 1j * rng.standard_normal(2)
+        arrays.append(a)
+    return arrays
+
+
+def _isometry_error(a, side):
+    cl, d, cr = a.shape
+    if side == "left":
+        m = a.reshape(cl * d, cr)  # rows (l, o), columns r  (internal layout (l, o, r))
+        return np.max(np.abs(m.conj().T @ m - np.eye(cr)))
+    m = a.reshape(cl, d * cr)
+    return np.max(np.abs(m @ m.conj().T - np.eye(cl)))
+
+
+@pytest.mark.parametrize("maxdim", [None, 6])
+def test_zero_singular_values_keep_isometric_factors(qh, maxdim):
+    """Kept singular values at the Jacobi's deflation floor (here exact zeros of a
+    padded product state) get an orthonormal completion of the singular vectors,
+    as LAPACK returns: the new sites stay isometries and the state is unchanged.
+    maxdim = None keeps all 8 columns of the 8 x 8 θ (6 of them zero);
+    maxdim = 6 keeps the default-threshold (1e-16) survivors."""
+    rng = np.random.default_rng(31)
+    n = 4
+    arrays = _padded_product_state(rng, n, 4)
+    psi = qh.from_internal(arrays)
+    ref = oc.Chain([a.copy() for a in arrays])
+    g = wl.haar_gate4(rng)
+    info = {}
+    qh.evolve_(psi, qh.gate(g, 2, 3), maxdim=maxdim, info=info)
+    oc.evolve(ref, oc.gate2(g, 2), maxdim=maxdim)
+    if maxdim is None:
+        assert info["kept"] == 8
+        assert psi.bond_dims() == ref.bond_dims()
+    assert np.max(np.abs(_state(psi) - oc.statevector(ref))) < 1e-12
+    arrs = psi.internal_arrays()
+    assert _isometry_error(arrs[1], "left") < 1e-12
+    assert _isometry_error(arrs[2], "right") < 1e-12
+    assert qh.isleftcanonical(psi, 2) and qh.isrightcanonical(psi, 3)
+
+
+@pytest.mark.parametrize("direction", ["right", "left"])
+def test_canonize_site_svd_on_padded_product_state(qh, direction):
+    """`canonize_site!(ψ, site; method = :svd)` on a rank-1 site matrix padded to
+    bond dimension 4: the factor left on the site is an isometry although three of
+    its singular values are zero (`Chain_test.jl:268-306` canonicality, atol 1e-12)."""
+    rng = np.random.default_rng(32)
+    arrays = _padded_product_state(rng, 4, 4)
+    psi = qh.from_internal(arrays)
+    v0 = oc.statevector(oc.Chain([a.copy() for a in arrays]))
+    site = 2 if direction == "right" else 3
+    qh.canonize_site_(psi, site, direction=direction, method="svd")
+    pred = qh.isleftcanonical if direction == "right" else qh.isrightcanonical
+    assert pred(psi, site)
+    assert _isometry_error(psi.internal_arrays()[site - 1], "left" if direction == "right" else "right") < 1e-12
+    assert np.max(np.abs(_state(psi) - v0)) < 1e-12 * np.max(np.abs(v0))
 
 
 def test_batched_layer_with_multichunk_descriptor_arena(qh, monkeypatch):

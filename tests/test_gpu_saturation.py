@@ -1,0 +1,122 @@
+"""Parity of the HIP path where the workloads' bond dimensions saturate.
+
+* Workload 2 (n = 64, Ising Trotter, cut-off 1e-12, maxdim 256) through 6 Strang
+  steps: χ reaches 256 at layer 14, so the cut-off decides the kept dimensions
+  while the state grows and the cap decides them afterwards.  Per call and
+  batched, against the oracle's fixture (`tests/golden/config2_saturation.npz`):
+  identical kept dimension for every one of the 570 updates, Λ of three bulk
+  bonds, every unnormalised ⟨Z_i⟩, ⟨ψ|ψ⟩ and the overlap with the initial state
+  within 1e-10.
+* A reduced batched uneven workload (16 MPS, n ~ U{8..20}, χmax in {16..256},
+  4 Haar layers, every layer of every MPS in ONE batched call) against the
+  oracle's fixture (`tests/golden/config3_reduced.npz`): FP64 within 1e-10,
+  the FP32-MFMA θ form within the bound stated below; kept dimensions identical.
+* The mode-A Λ rule (`src/Ansatz/Chain.jl:616`: θ contracts the bond's own Λ,
+  not its neighbours') pinned by a mathematical identity: in a state whose
+  environment is isometric on both sides of the bond -- with non-trivial Λ's on
+  the neighbouring bonds -- the discarded norm of a truncated update equals
+  ‖Gψ − ψ_trunc‖ (Eckart–Young on the Schmidt matrix).  A neighbouring Λ
+  entering θ breaks the identity (checked on the oracle in test_oracle.py).
+"""
+import numpy as np
+import pytest
+
+from oracle import chain as oc
+from oracle import statevector as sv
+from oracle import workloads as wl
+from tests.golden import make_golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+# FP32 θ (v_mfma_f32_16x16x4_f32, FP32 accumulation) against the FP64 oracle on the
+# reduced batched workload, on unnormalised <Z_i> of normalised states and on the
+# overlap with the initial state; the SVD and truncation stay FP64.
+FP32_BOUND = 1e-5
+
+
+def _z_terms(qh, n):
+    return [[qh.gate(wl.Z, i)] for i in range(1, n + 1)]
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_config2_through_saturation_matches_golden(qh, batched):
+    fx = np.load(make_golden.path("config2_saturation"), allow_pickle=False)
+    arrays, layers, tr, _ = wl.config2(steps=int(fx["steps"]))
+    psi = qh.from_internal(arrays)
+    psi0 = psi.copy()
+    kept = []
+    for layer in layers:
+        if batched:
+            k, _ = qh.evolve_layer_([(psi, qh.gate(g, b, b + 1)) for b, g in layer], **tr)
+            kept.extend(k)
+        else:
+            for b, g in layer:
+                info = {}
+                qh.evolve_(psi, qh.gate(g, b, b + 1), info=info, **tr)
+                kept.append(info["kept"])
+    kept = np.array(kept)
+    margin = fx["threshold_margin"]
+    print(f"closest cut-off decision: |log(sigma / 1e-12)| = {np.min(margin):.2e}")
+    bad = np.nonzero(kept != fx["kept"])[0]
+    assert bad.size == 0, [(int(i), int(kept[i]), int(fx["kept"][i]), float(margin[i])) for i in bad]
+    assert psi.bond_dims() == list(fx["final_dims"])
+    assert max(psi.bond_dims()) == 256
+    for b in fx["lambda_bonds"]:
+        lam = psi.lambda_(int(b))
+        assert np.max(np.abs(lam - fx[f"lambda{b}"])) < TOL
+    z = qh.expect_many(psi, _z_terms(qh, psi.n))
+    dz = np.max(np.abs(z - fx["z"]))
+    print(f"max |<Z_i> - <Z_i>_oracle| = {dz:.2e}")
+    assert dz < TOL
+    assert abs(qh.overlap(psi, psi) - fx["norm2"]) < TOL
+    assert abs(qh.overlap(psi, psi0) - fx["overlap"]) < TOL
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", TOL), ("fp32", FP32_BOUND)])
+def test_config3_reduced_batched_matches_golden(qh, precision, bound):
+    fx = np.load(make_golden.path("config3_reduced"), allow_pickle=False)
+    items = wl.config3(**make_golden.CONFIG3_REDUCED)
+    assert int(fx["count"]) == len(items)
+    states = [qh.from_internal(it["arrays"]) for it in items]
+    starts = [s.copy() for s in states]
+    kept = [[] for _ in items]
+    depth = len(items[0]["layers"])
+    for layer in range(depth):
+        ups, mds, owner = [], [], []
+        for m, (it, psi) in enumerate(zip(items, states)):
+            for b, g in it["layers"][layer]:
+                ups.append((psi, qh.gate(g, b, b + 1)))
+                mds.append(it["chi"])
+                owner.append(m)
+        k, _ = qh.evolve_layer_(ups, maxdim=mds, precision=precision)  # every MPS in one call
+        for m, kk in zip(owner, k):
+            kept[m].append(kk)
+    worst_z = worst_ov = 0.0
+    for m, (it, psi) in enumerate(zip(items, states)):
+        assert np.array_equal(np.array(kept[m]), fx[f"kept{m}"]), m
+        z = qh.expect_many(psi, _z_terms(qh, psi.n))
+        worst_z = max(worst_z, float(np.max(np.abs(z - fx[f"z{m}"]))))
+        worst_ov = max(worst_ov, abs(qh.overlap(psi, starts[m]) - fx[f"overlap{m}"]))
+    print(f"{precision}: max |<Z_i> - oracle| = {worst_z:.2e}, max |overlap - oracle| = {worst_ov:.2e}")
+    assert worst_z < bound
+    assert worst_ov < bound
+    if precision != "fp64":
+        assert worst_z > 0.0  # the reduced-precision θ did run
+
+
+def test_mode_a_discarded_norm_is_state_error(qh):
+    rng = np.random.default_rng(41)
+    bond, keep = 4, 3
+    arrays, lams = wl.isometric_environment_state(rng, bond=bond)
+    g = wl.haar_gate4(rng)
+    v_exact = sv.apply2(oc.statevector(oc.Chain([a.copy() for a in arrays], [l for l in lams])), g, bond)
+    psi = qh.from_internal(arrays, lambdas=lams)
+    info = {}
+    qh.evolve_(psi, qh.gate(g, bond, bond + 1), maxdim=keep, info=info)
+    assert info["kept"] == keep
+    v_trunc = oc.statevector(oc.Chain(psi.internal_arrays(), psi.lambdas()))
+    err = np.linalg.norm(v_exact - v_trunc)
+    print(f"discarded {info['discarded']:.6e}, |G psi - psi_trunc| {err:.6e}")
+    assert info["discarded"] > 1e-3
+    assert abs(err - info["discarded"]) < 1e-10 * err

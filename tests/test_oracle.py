@@ -105,6 +105,42 @@ def test_mode_a_theta_includes_own_lambda_only():
     assert np.allclose(oc.statevector(psi), sv.apply2(v, g, 3))
 
 
+def test_mode_a_truncation_error_equals_discarded_norm():
+    """Pins which Λ's enter the mode-A θ (`Chain.jl:616`) by an identity: when the
+    environment of the bond is isometric on both sides (sites left of it
+    left-isometric including the Λ on the bond before, sites right of it
+    right-isometric including the Λ on the bond after), the θ of the bond's own
+    tensors is the Schmidt matrix, so the truncated update moves the gated state by
+    exactly the Eckart–Young norm of the dropped singular values.  The control: a θ
+    that also absorbs the neighbouring Λ's (a plausible mistake) does not satisfy
+    the identity on the same state."""
+    rng = np.random.default_rng(41)
+    bond, keep = 4, 3
+    arrays, lams = wl.isometric_environment_state(rng, bond=bond)
+    psi = oc.Chain([a.copy() for a in arrays], [None if l is None else l.copy() for l in lams])
+    g = wl.haar_gate4(rng)
+    v_exact = sv.apply2(oc.statevector(psi), g, bond)
+    th = oc.theta(psi, bond, g)
+    cl, d, _, cr = th.shape
+    s = np.linalg.svd(th.reshape(cl * d, d * cr), compute_uv=False)
+    oc.evolve(psi, oc.gate2(g, bond), maxdim=keep)
+    err = np.linalg.norm(v_exact - oc.statevector(psi))
+    assert err > 1e-3
+    assert err == pytest.approx(np.sqrt(np.sum(s[keep:] ** 2)), rel=1e-10)
+    # control: θ with the neighbouring Λ's absorbed, cut to the same rank and the
+    # Λ's divided back out afterwards
+    lo, hi = lams[bond - 2], lams[bond]
+    wrong = th * lo[:, None, None, None] * hi[None, None, None, :]
+    u, sw, vh = np.linalg.svd(wrong.reshape(cl * d, d * cr), full_matrices=False)
+    cut = (u[:, :keep] * sw[:keep]) @ vh[:keep]
+    cut = cut.reshape(cl, d, d, cr) / lo[:, None, None, None] / hi[None, None, None, :]
+    full = th.reshape(cl, d, d, cr)
+    # the state error of that cut is the θ-space error weighted by the isometric
+    # environment, i.e. the Frobenius norm of the change of the bond's own θ
+    err_wrong = np.linalg.norm(cut - full)
+    assert abs(err_wrong - np.sqrt(np.sum(sw[keep:] ** 2))) > 1e-3 * err_wrong
+
+
 def test_vidal_mode_exact_without_truncation():
     rng = np.random.default_rng(5)
     psi = rand_chain(rng, 7, chi=8)
