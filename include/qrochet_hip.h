@@ -183,6 +183,14 @@ int qh_expect_many(qh_context* ctx, qh_mps* mps, int nterms, const int* offsets,
 int qh_overlap(qh_context* ctx, qh_mps* a, qh_mps* b, double* out);
 /* `norm(ψ)` (src/Ansatz.jl:101-109). */
 int qh_norm(qh_context* ctx, qh_mps* mps, double* out);
+/* qh_overlap without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The environments live in a fixed
+ * 64 MiB device array per GPU, set up when a context is created; open chains whose
+ * sweep needs more (2 chi_b*chi_a + chi_b*d*chi_a complex words at the widest
+ * bond, about chi <= 1024 at d = 2) get QH_ERR_ARGUMENT.  Calls that overlap in time
+ * on one GPU share that array and race: the caller serialises them. */
+int qh_overlap_noalloc(qh_context* ctx, qh_mps* a, qh_mps* b, double* out);
 
 /* ---- MPO × MPS ------------------------------------------------------------ */
 /* Replaces `MPO(arrays)` = `Chain(Operator(), Open(), arrays)` (src/Ansatz/Chain.jl:20,133-172)

@@ -133,14 +133,13 @@ int theta_tile_extent(int d) { return d == 2 ? 32 : 16; }
 // ---------------------------------------------------------------------------
 // batched GEMM  C = alpha op(A) diag(dk) op(B) (+ beta C), then row/col scales
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ items, const int2* __restrict__ tiles) {
+// one 64 × 64 output tile of item g
+__device__ __forceinline__ void gemm_tile_f64(const GemmItem& g, int tile) {
     constexpr int TM = GEMM_TM, TN = GEMM_TN, TK = GEMM_TK;
     __shared__ double2 sA[TK][TM + 1];
     __shared__ double2 sB[TK][TN + 1];
     const int tid = threadIdx.x;
-    const int2 tt = tiles[blockIdx.x];
-    const GemmItem g = items[tt.x];
-    const int m0 = (tt.y % g.tiles_m) * TM, n0 = (tt.y / g.tiles_m) * TN;
+    const int m0 = (tile % g.tiles_m) * TM, n0 = (tile / g.tiles_m) * TN;
     const int tm = tid & 15, tn = tid >> 4;
 
     double2 acc[4][4];
@@ -209,9 +208,28 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ i
     }
 }
 
+__global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ items, const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const GemmItem g = items[tt.x];
+    gemm_tile_f64(g, tt.y);
+}
+
+// one GEMM whose descriptor travels as a kernel argument (no descriptor upload)
+__global__ __launch_bounds__(256) void k_gemm_f64_arg(GemmItem g) { gemm_tile_f64(g, blockIdx.x); }
+
 void launch_gemm(const GemmItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
     if (ntiles <= 0) return;
     k_gemm_f64<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
+void launch_gemm_arg(GemmItem g, hipStream_t s) {
+    g.tiles_m = (g.m + GEMM_TM - 1) / GEMM_TM;
+    g.tiles_n = (g.n + GEMM_TN - 1) / GEMM_TN;
+    if (g.dk_mod <= 0) g.dk_mod = 1;
+    if (g.dk_div <= 0) g.dk_div = 1;
+    const int ntiles = g.tiles_m * g.tiles_n;
+    if (ntiles <= 0) return;
+    k_gemm_f64_arg<<<ntiles, 256, 0, s>>>(g);
 }
 
 }  // namespace qh

@@ -769,6 +769,7 @@ int qh_ctx_create(int device, qh_context** out) {
         QH_HIP(hipGetDeviceCount(&ndev));
         if (device < 0 || device >= ndev) throw Error(QH_ERR_ARGUMENT, "invalid device ordinal");
         QH_HIP(hipSetDevice(device));
+        noalloc_setup(device);  // this device's scratch address for qh_overlap_noalloc
         c = new qh_context();
         c->device = device;
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

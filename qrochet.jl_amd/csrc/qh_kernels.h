@@ -71,6 +71,8 @@ __device__ __forceinline__ cplx load_op(const cplx* x, int ld, int r, int c, int
 }
 // FP64 on the vector ALUs
 void launch_gemm(const GemmItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// the same kernel for one GEMM passed by value (tiles and divisors filled in here)
+void launch_gemm_arg(GemmItem g, hipStream_t s);
 // FP32 or bf16x3 products on MFMA with FP32 accumulation (precision = QH_PREC_FP32 /
 // QH_PREC_BF16X3), FP64 epilogue; same tiles as launch_gemm
 void launch_gemm_mfma(const GemmItem* d_items, const int2* d_tiles, int ntiles, int precision, hipStream_t s);

@@ -139,6 +139,10 @@ struct Timed {
     ~Timed();
 };
 
+// With `device` current, once per device: the device address of the fixed scratch
+// array of qh_overlap_noalloc (throws on failure).
+void noalloc_setup(int device);
+
 // ---- batched building blocks ----
 void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_PREC_FP64);
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);

@@ -31,6 +31,7 @@ from .chain import (  # noqa: F401
     norm,
     normalize_,
     overlap,
+    overlap_noalloc,
     pMPS,
     simple_update_,
     site,

@@ -82,6 +82,7 @@ SIGNATURES = {
     "qh_expect_many": (C.c_int, [_ctx_p, _mps_p, C.c_int, _ip, C.POINTER(QHObservable), _dp]),
     "qh_overlap": (C.c_int, [_ctx_p, _mps_p, _mps_p, _dp]),
     "qh_norm": (C.c_int, [_ctx_p, _mps_p, _dp]),
+    "qh_overlap_noalloc": (C.c_int, [_ctx_p, _mps_p, _mps_p, _dp]),
     "qh_mpo_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mpo_p)]),
     "qh_pmpo_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mpo_p)]),
     "qh_mpo_free": (C.c_int, [_mpo_p]),

@@ -653,6 +653,14 @@ def overlap(a: Chain, b: Chain) -> complex:
     return complex(out[0], out[1])
 
 
+def overlap_noalloc(a: Chain, b: Chain) -> complex:
+    """`overlap(a, b)` with no device allocation (`qh_overlap_noalloc`: the
+    environments live in the library's fixed per-GPU scratch; open chains only)."""
+    out = (C.c_double * 2)()
+    a.ctx.check(L.load().qh_overlap_noalloc(a.ctx.handle, a.handle, b.handle, out))
+    return complex(out[0], out[1])
+
+
 def norm(psi: Chain, p=2) -> float:
     """`norm(ψ, p=2)` (src/Ansatz.jl:101-109)."""
     if p != 2:
