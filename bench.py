@@ -16,11 +16,11 @@ Printed JSON: updates/s over the timed steps, plus
   * the CPU oracle (numpy/LAPACK zgesdd restatement) on a bounded sample of the
     same saturated layers (`cpu_baseline`),
   * the request loop (`serving`): Poisson arrivals at a fixed QPS of brickwork
-    Haar-circuit requests, p50/p99 latency and updates/s.  The default run uses
-    a bounded sample of the serving distribution (n ~ U{8..64}, χmax
-    log-uniform in {32..256}, depth ~ U{1..40}); `--serving full
-    --serving-only` runs the full one (n ~ U{8..128}, χmax log-uniform in
-    {32..1024}) and prints its own line.
+    Haar-circuit requests, p50/p99 latency and updates/s.  The default run draws
+    12 requests at 0.1 QPS from the full serving distribution (n ~ U{8..128},
+    χmax log-uniform in {32..1024}, depth ~ U{1..40}); `--serving sample` uses a
+    lighter one (n ~ U{8..64}, χmax up to 256) and `--serving-only` prints the
+    leg as its own line.
 """
 from __future__ import annotations
 
@@ -140,6 +140,11 @@ def batched_leg(qh, wl, precision, count, depth):
     items = wl.config3(count=count, depth=depth)
     log(f"batched workload: {count} MPS generated")
     ctx = qh.default_context()
+    # the SVD schedule (Jacobi block width, QR passes per size class) by the tuner
+    t_tune = time.perf_counter()
+    tuned = ctx.autotune(max_q=512, batch=32)
+    t_tune = time.perf_counter() - t_tune
+    log(f"autotune: {tuned} ({t_tune:.1f} s)")
     states = [qh.from_internal(it["arrays"]) for it in items]
     ctx.synchronize()
     n_up = sum(len(l) for it in items for l in it["layers"])
@@ -167,6 +172,7 @@ def batched_leg(qh, wl, precision, count, depth):
                                f"maxdim=chimax, threshold default", "precision": precision,
                    "updates": n_up, "wall_s": round(elapsed, 3)},
         "kernel_ms": {k: round(ctx.timing(k)["ms"], 2) for k in classes},
+        "autotune": {"classes": tuned, "seconds": round(t_tune, 2)},
     }
 
 
@@ -321,13 +327,16 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU-baseline repetitions (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serving", choices=("sample", "full", "off"), default="sample")
+    ap.add_argument("--serving", choices=("sample", "full", "off"), default="full",
+                    help="request-loop leg: the full serving distribution (default: 12 requests at 0.1 QPS), "
+                         "a lighter sample (n <= 64, chimax <= 256), or none")
     ap.add_argument("--serving-only", action="store_true", help="run only the request loop, print its own line")
     ap.add_argument("--serving-requests", type=int, default=None)
     ap.add_argument("--qps", type=float, default=None)
-    ap.add_argument("--workload", choices=("tebd", "batched", "mpo"), default="tebd",
+    ap.add_argument("--workload", choices=("tebd", "batched", "mpo", "contractions"), default="tebd",
                     help="tebd: the n=64, chi=256 Trotter evolution (default); batched: 512 uneven MPS; "
-                         "mpo: MPO application + variational compression precision study")
+                         "mpo: MPO application + variational compression precision study; contractions: "
+                         "expect_many / overlap / norm / canonize! on the chi=256 workload state")
     ap.add_argument("--mpo-n", type=int, default=32)
     ap.add_argument("--mpo-chi", type=int, default=512)
     ap.add_argument("--mpo-D", type=int, default=8)
@@ -365,6 +374,19 @@ def main():
                                  args.mpo_cpu_chi)))
         return
 
+    if args.workload == "contractions":
+        from qrochet_hip import workloads as wl
+
+        arrays, layers, trunc, _ = wl.config2(steps=args.trotter)
+        psi, _ = run_workload(qh, arrays, layers, trunc, stop_layer=min(30, len(layers)))
+        out = contractions_leg(qh, wl, psi)
+        print(json.dumps({"metric": "expect_many (all Z_i and Z_iZ_j, |i-j|<=4) / overlap / norm / canonize! "
+                                    "wall time on the workload-2 state after 10 Trotter steps",
+                          "value": out["expect_many_ms"], "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+                          "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+                          "config": {"workload": out["state"]}, "contractions": out}))
+        return
+
     if args.workload == "batched":
         from qrochet_hip import workloads as wl
 
@@ -373,7 +395,7 @@ def main():
 
     if args.serving != "off":
         full = args.serving == "full"
-        n_req = args.serving_requests or (40 if full else 48)
+        n_req = args.serving_requests or (12 if full else 48)
         qps = args.qps or (0.1 if full else 4.0)
     if args.serving_only:
         if args.serving == "off":
@@ -510,6 +532,13 @@ def main():
         tp = time.perf_counter() - tp0
         extra["saturated_plain_fp64_updates_per_s"] = round(sample_updates / tp, 2)
         extra["batched_over_plain"] = round(tp / tb, 3)
+        # the mixed-precision target: batched FP32-contraction updates against the plain
+        # (per-call, unbatched) FP64 form, at <= 1e-6 error in <Z_i>
+        m32 = mixed["fp32"]
+        extra["mixed_precision_target"] = {
+            "mixed_over_plain_fp64": round(m32["updates_per_s"] * tp / sample_updates, 3),
+            "max_abs_dZ_vs_fp64": m32["max_abs_dZ_vs_fp64"], "bound": 1e-6, "speedup_goal": 5.0,
+            "met": bool(m32["updates_per_s"] * tp / sample_updates >= 5.0 and m32["max_abs_dZ_vs_fp64"] <= 1e-6)}
         extra["saturated_sample"] = f"{args.sample_layers} layers ({sample_updates} updates) at chi<=256 after 10 Trotter steps"
 
     log("plain sample done")

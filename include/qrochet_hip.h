@@ -81,13 +81,31 @@ int qh_ctx_set_timing(qh_context* ctx, int enable);
 int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_ms, int64_t* launches,
                       double* algorithmic_flops, double* algorithmic_bytes);
 int qh_ctx_reset_timing(qh_context* ctx);
-/* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default. */
+/* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default.  Matrices
+ * still rotating at the limit get a second window of as many sweeps before the call
+ * returns QH_ERR_NOT_CONVERGED. */
 int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);
 /* Householder QR passes that precondition the Jacobi SVD of matrices with more
  * than one block pair (0..4, default 3): X = Q1 R1, R_{j-1}^H = Q_j R_j, Jacobi on
  * R_n^H.  Changes the rounding of the results only, not their meaning (no
  * reference counterpart: a tuning knob of this SVD). */
 int qh_ctx_set_qr_passes(qh_context* ctx, int passes);
+/* HIP graphs for the Jacobi SVD (no reference counterpart): with enable = 1 (the
+ * default) the rounds of a sweep are captured once per batched SVD and replayed as
+ * one graph launch per sweep; 0 launches them one by one.  The results are the same. */
+int qh_ctx_set_graphs(qh_context* ctx, int enable);
+/* Schedule tuner of the batched SVD (no reference counterpart; the project's
+ * autotuner): for each size class q = 64, 128, ... <= max_q (at most 4096) it times
+ * the batched SVD of `batch` synthetic q × q matrices (flat and decaying spectra;
+ * fewer above q = 512) under every candidate schedule -- Jacobi block width 8 or 16
+ * and 1..4 QR preconditioning passes -- and keeps the fastest per class for this
+ * context: a matrix then gets the passes of the smallest class >= its column count,
+ * and a batch the block width of its largest member's class.  report (may be NULL):
+ * 5 doubles per class (q, block, passes, best ms, ms of the previous setting), at most
+ * 8 classes; nclasses (may be NULL) receives their number.  qh_ctx_set_jacobi and
+ * qh_ctx_set_qr_passes override the tuned values; qh_ctx_autotune_reset drops them. */
+int qh_ctx_autotune(qh_context* ctx, int max_q, int batch, double* report, int* nclasses);
+int qh_ctx_autotune_reset(qh_context* ctx);
 /* Device memory of the context's GPU (no reference counterpart: the request loop
  * sizes its admission budget from total_bytes). */
 int qh_ctx_mem_info(qh_context* ctx, size_t* free_bytes, size_t* total_bytes);
@@ -108,6 +126,13 @@ int qh_mps_create(qh_context* ctx, int n, int d, const int* bond_dims, const dou
  * (trace over the wrap bond, cost O(n·W²·χ³) for wrap dimension W); the methods the
  * reference defines for `Open` only return QH_ERR_ARGUMENT. */
 int qh_pmps_create(qh_context* ctx, int n, int d, const int* bond_dims, const double* const* arrays, qh_mps** out);
+/* `rand(Chain, Open, State; n, χ, p, eltype)` (src/Ansatz/Chain.jl:223-256): a random
+ * normalised open MPS with bond b of dimension min(χ, p^b, p^(n-b)), right-canonical
+ * from site 2 (orthonormalised rows of uniform [0, 1) entries), site 1 an isometry
+ * divided by sqrt(p).  complex_entries: 0 for eltype Float64, 1 for ComplexF64.  The
+ * entries come from a counter-based hash of (seed, site, entry): the same seed gives
+ * the same chain; Julia's RNG stream is not reproduced. */
+int qh_mps_rand(qh_context* ctx, int n, int chi, int p, int complex_entries, unsigned long long seed, qh_mps** out);
 /* periodic: 1 for a periodic chain, 0 for an open one; wrap_dim: the bond between
  * site n and site 1 (1 for an open chain) */
 int qh_mps_is_periodic(const qh_mps* mps, int* periodic, int* wrap_dim);

@@ -94,7 +94,8 @@ __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j
 template <int BC>
 // three waves per SIMD: the MFMA accumulators fit in the VGPRs without spilling
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_jacobi_round(const JacItem* __restrict__ items,
-                                                      const int2* __restrict__ tiles, int round, int sort_sweeps,
+                                                      const int2* __restrict__ tiles, int round_arg,
+                                                      const int* __restrict__ round_base, int sort_sweeps,
                                                       int debug, int max_inner) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
@@ -119,6 +120,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int2 tt = tiles[blockIdx.x];
     const JacItem it = items[tt.x];
     if (*((volatile int*)it.conv)) return;
+    // the round of the tournament: an argument, or the sweep's first round kept in device
+    // memory plus the argument (so that a captured sweep replays with advancing rounds)
+    const int round = round_base ? *round_base + round_arg : round_arg;
     const int R = it.nb - 1;
     const int r = round % R;
     const double tol = it.tol;
@@ -564,13 +568,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
 }
 
-void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         int sort_sweeps, int debug, int max_inner, hipStream_t s) {
+void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
+                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
-        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, sort_sweeps, debug, max_inner);
+        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
     else
-        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, sort_sweeps, debug, max_inner);
+        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
 }
 
 // ||X||_F^2 of every item, deterministic: each workgroup reduces one chunk into
@@ -627,7 +631,8 @@ void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int 
 // After a window of rounds: an item with no rotation is converged.  remaining[0]
 // receives the number of items still rotating, remaining[1] the number with
 // non-finite entries.
-__global__ void k_conv_update(const JacItem* __restrict__ items, int count, int* remaining) {
+__global__ void k_conv_update(const JacItem* __restrict__ items, int count, int* remaining, int* round_base,
+                              int rounds) {
     __shared__ int sRem, sBad;
     if (threadIdx.x == 0) sRem = sBad = 0;
     __syncthreads();
@@ -646,10 +651,12 @@ __global__ void k_conv_update(const JacItem* __restrict__ items, int count, int*
     if (threadIdx.x == 0) {
         remaining[0] = sRem;
         remaining[1] = sBad;
+        if (round_base) *round_base += rounds;  // the next sweep's first round
     }
 }
-void launch_conv_update(const JacItem* d_items, int count, int* remaining, hipStream_t s) {
-    k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining);
+void launch_conv_update(const JacItem* d_items, int count, int* remaining, int* round_base, int rounds,
+                        hipStream_t s) {
+    k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining, round_base, rounds);
 }
 
 // ---------------------------------------------------------------------------
@@ -832,6 +839,22 @@ __global__ __launch_bounds__(256) void k_complete(const CompleteItem* __restrict
 
 void launch_complete(const CompleteItem* d_items, int count, hipStream_t s) {
     if (count > 0) k_complete<<<count, 256, 0, s>>>(d_items);
+}
+
+__global__ __launch_bounds__(256) void k_scatter_cols(const ScatterItem* __restrict__ items,
+                                                      const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const ScatterItem it = items[tt.x];
+    const int j = tt.y;
+    if (j >= *it.kept) return;
+    const double sg = it.sigma_sorted[j];
+    const double f = (it.mul_sigma && sg > 0.0) ? sg : 1.0;
+    double2* dst = it.x + (size_t)it.p * it.perm[j];
+    const double2* src = it.f + (size_t)it.ldf * j;
+    for (int r = threadIdx.x; r < it.p; r += 256) dst[r] = cscale(src[r], f);
+}
+void launch_scatter_cols(const ScatterItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles > 0) k_scatter_cols<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 
 // ---------------------------------------------------------------------------

@@ -47,6 +47,33 @@ void launch_apply_op(double2* t, int left, int dim, int right, const double2* op
     }
 }
 
+// The same for a batch of tensors with their own shapes and operators (D <= 16, read
+// at run time); tile map: (item, block of 256 (l, r) pairs).
+__global__ __launch_bounds__(256) void k_apply_op_batch(const ApplyItem* __restrict__ items,
+                                                        const int2* __restrict__ tiles) {
+    __shared__ double2 sOp[256];
+    const int2 tt = tiles[blockIdx.x];
+    const ApplyItem it = items[tt.x];
+    const int D = it.dim;
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) sOp[i] = it.op[i];
+    __syncthreads();
+    const size_t e = (size_t)tt.y * 256 + threadIdx.x;
+    const size_t total = (size_t)it.left * it.right;
+    if (e >= total) return;
+    const size_t l = e % it.left, r = e / it.left;
+    double2* base = it.t + l + (size_t)it.left * D * r;
+    double2 in[16];
+    for (int v = 0; v < D; ++v) in[v] = base[(size_t)it.left * v];
+    for (int u = 0; u < D; ++u) {
+        double2 acc = cz();
+        for (int v = 0; v < D; ++v) acc = cfma(sOp[u + D * v], in[v], acc);
+        base[(size_t)it.left * u] = acc;
+    }
+}
+void launch_apply_op_batch(const ApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles > 0) k_apply_op_batch<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
 // A[l, o, r] *= f(λ[l]) (mode 0) or f(λ[r]) (mode 2);  f = pinv ? (|λ|>atol ? 1/λ : 0) : λ
 __global__ void k_scale_bond(double2* __restrict__ a, int chil, int d, int chir, const double* __restrict__ lam,
                              int mode, int pinv, double atol) {
@@ -157,6 +184,35 @@ void launch_env_close(const cplx* e, const cplx* r, int nb, int na, const double
                       hipStream_t s) {
     k_env_close<<<1, 1024, 0, s>>>(e, r, nb, na, lb, la, out);
 }
+// one workgroup per item, the same reduction order as k_env_close
+__global__ __launch_bounds__(1024) void k_env_close_batch(const CloseItem* __restrict__ items) {
+    const CloseItem it = items[blockIdx.x];
+    __shared__ double sRe[16], sIm[16];
+    double2 acc = cz();
+    const size_t total = (size_t)it.nb * it.na;
+    for (size_t k = threadIdx.x; k < total; k += blockDim.x) {
+        const int b = (int)(k % it.nb), a = (int)(k / it.nb);
+        double2 v = cmul(it.e[k], it.r[a + (size_t)it.na * b]);
+        double f = 1.0;
+        if (it.lb) f *= it.lb[b];
+        if (it.la) f *= it.la[a];
+        acc = cadd(acc, cscale(v, f));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, 64);
+        acc.y += __shfl_xor(acc.y, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) { sRe[threadIdx.x >> 6] = acc.x; sIm[threadIdx.x >> 6] = acc.y; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 t = cz();
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { t.x += sRe[w]; t.y += sIm[w]; }
+        *it.out = t;
+    }
+}
+void launch_env_close_batch(const CloseItem* d_items, int count, hipStream_t s) {
+    if (count > 0) k_env_close_batch<<<count, 1024, 0, s>>>(d_items);
+}
 
 __global__ void k_fill(double* p, size_t n, double v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,6 +221,53 @@ __global__ void k_fill(double* p, size_t n, double v) {
 void launch_fill(double* p, size_t n, double v, hipStream_t s) {
     if (n == 0) return;
     k_fill<<<(int)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+}
+
+// Uniform [0, 1) entries for the sampler of `rand(Chain, Open, State)`
+// (src/Ansatz/Chain.jl:223-256): entry e of buffer `key` is a splitmix64 hash of
+// (seed, key, e), real part and -- for a complex element type -- imaginary part.
+__device__ __forceinline__ double splitmix01(unsigned long long k) {
+    k += 0x9e3779b97f4a7c15ull;
+    k = (k ^ (k >> 30)) * 0xbf58476d1ce4e5b9ull;
+    k = (k ^ (k >> 27)) * 0x94d049bb133111ebull;
+    k ^= k >> 31;
+    return (double)(k >> 11) * 0x1.0p-53;
+}
+__global__ void k_rand_uniform(double2* p, size_t n, unsigned long long seed, unsigned long long key, int complex_entries) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long base = seed ^ (key * 0xd1b54a32d192ed03ull);
+    const double re = splitmix01(base + 2 * (unsigned long long)i);
+    const double im = complex_entries ? splitmix01(base + 2 * (unsigned long long)i + 1) : 0.0;
+    p[i] = make_double2(re, im);
+}
+void launch_rand_uniform(cplx* p, size_t n, unsigned long long seed, unsigned long long key, int complex_entries,
+                         hipStream_t s) {
+    if (n == 0) return;
+    k_rand_uniform<<<(int)((n + 255) / 256), 256, 0, s>>>(p, n, seed, key, complex_entries);
+}
+
+__global__ void k_scale_real(double2* p, size_t n, double f) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(p[i].x * f, p[i].y * f);
+}
+void launch_scale_real(cplx* p, size_t n, double f, hipStream_t s) {
+    if (n == 0) return;
+    k_scale_real<<<(int)((n + 255) / 256), 256, 0, s>>>(p, n, f);
+}
+
+// column j of a rows × cols matrix times 10^(-decades · j / (cols - 1))
+__global__ void k_scale_cols_geometric(double2* p, int rows, int cols, double decades) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    const int j = (int)(i / rows);
+    const double f = cols > 1 ? exp10(-decades * j / (cols - 1)) : 1.0;
+    p[i] = make_double2(p[i].x * f, p[i].y * f);
+}
+void launch_scale_cols_geometric(cplx* p, int rows, int cols, double decades, hipStream_t s) {
+    const size_t t = (size_t)rows * cols;
+    if (t == 0) return;
+    k_scale_cols_geometric<<<(int)((t + 255) / 256), 256, 0, s>>>(p, rows, cols, decades);
 }
 
 // identity matrix (n × n, ld n)

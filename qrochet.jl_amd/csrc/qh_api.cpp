@@ -2,10 +2,12 @@
 // Each entry point restates one method of src/Ansatz/Chain.jl (cited per
 // function) on device-resident tensors; see include/qrochet_hip.h.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <utility>
@@ -122,6 +124,7 @@ void evolve2_core(qh_context* c, int count, qh_mps* const* ms, const int* bonds,
         s.threshold = std::isnan(threshold) ? 1e-16 : threshold;
         s.maxdim = maxdim;
         s.renormalize = (canonical && renorm && trunc) ? 1 : 0;
+        s.refine = canonical ? 1 : 0;  // Vidal form: the factors are the canonical tensors
         const int kmax = s.kmax();
         nl[i] = dalloc<cplx>(c, (size_t)chil * d * kmax);
         nr[i] = dalloc<cplx>(c, (size_t)kmax * d * chir);
@@ -245,6 +248,7 @@ void canonize_site_core(qh_context* c, qh_mps* m, int site, int direction, int m
     SvdTask t;
     t.x = x; t.mr = mr; t.mc = mc;
     t.truncate = 0;
+    t.refine = 1;  // the site must pass isleft/isrightcanonical like LAPACK's factors
     t.lam_dst = sig;
     cplx* na = nullptr;   // new tensor at site
     cplx* f2 = nullptr;   // factor passed to the neighbour
@@ -366,10 +370,11 @@ void contract_between_core(qh_context* c, qh_mps* m, int s1, int s2, int directi
 }
 
 // Chain.jl:469-497
-void canonize_core(qh_context* c, qh_mps* m) {
-    check_open(m, "canonize!");
+// The left-to-right half of canonize! (Chain.jl:469-497) after the right-to-left QR
+// sweep, as the reference takes it: n-1 dependent SVDs, each absorbing the previous
+// one's s V^H.  Used when a bond is wider than d times its left neighbour.
+void canonize_svd_sweep(qh_context* c, qh_mps* m) {
     const int n = m->n;
-    for (int i = n; i >= 2; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
     std::vector<double*> lams(n - 1, nullptr);
     for (int i = 1; i <= n - 1; ++i) {
         canonize_site_core(c, m, i, QH_RIGHT, QH_METHOD_SVD);
@@ -381,6 +386,114 @@ void canonize_core(qh_context* c, qh_mps* m) {
         launch_scale_bond(m->site[i - 1], m->chi[i - 1], m->d, m->chi[i], lams[i - 2], 0, 1, 1e-64, c->stream);
         m->lam[i - 2] = lams[i - 2];
     }
+    ctx_sync(c);
+}
+
+// `canonize!(ψ)` (Chain.jl:469-497): Vidal form Γ_1 Λ_1 Γ_2 … Γ_n with the Schmidt
+// values on every bond.  After the reference's right-to-left QR sweep (sites 2..n
+// right isometries B_i), a left-to-right sweep of QRs M_i = Q_i R_i with
+// M_1 = A_1, M_i = R_{i-1} B_i gives ψ = Q_1⋯Q_i · R_i · B_{i+1}⋯B_n for every bond i,
+// an isometry on each side of R_i: the SVDs R_i = U_i s_i V_i^H are the Schmidt
+// decompositions the reference's dependent SVD sweep finds one after the other, and
+// here they are independent (one batched call).  Then Λ_i = s_i and
+//   Γ_1 = Q_1 U_1,   Γ_i = pinv(s_{i-1}) U_{i-1}^H Q_i U_i,   Γ_n = pinv(s_{n-1}) U_{n-1}^H M_n
+// (pinv with atol 1e-64, Chain.jl:491) -- the reference's tensors up to the gauge of
+// each singular subspace.  The QR sweep needs every M_i tall (χ_{i-1} d >= χ_i);
+// otherwise the reference's SVD sweep runs instead.
+void canonize_core(qh_context* c, qh_mps* m) {
+    check_open(m, "canonize!");
+    const int n = m->n, d = m->d;
+    for (int i = n; i >= 2; --i) canonize_site_core(c, m, i, QH_LEFT, QH_METHOD_QR);
+    for (int i = 1; i <= n - 1; ++i)
+        if (m->chi[i - 1] * d < m->chi[i]) {
+            canonize_svd_sweep(c, m);
+            return;
+        }
+    // left-to-right QR sweep: Q_i (χ_{i-1} d × χ_i), R_i^H (χ_i × χ_i)
+    std::vector<cplx*> Q(n + 1, nullptr), RH(n + 1, nullptr);
+    cplx* M = dcopy(c, m->site[0], site_elems(m, 1));
+    for (int i = 1; i <= n - 1; ++i) {
+        const int rows = m->chi[i - 1] * d, cols = m->chi[i];
+        const size_t tq = (size_t)((cols + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
+        cplx* v = dalloc<cplx>(c, (size_t)rows * cols);
+        cplx* t = dalloc<cplx>(c, tq);
+        std::vector<QrJob> jobs{QrJob{M, rows, cols, rows, v, t}};
+        run_qr_factor(c, jobs);
+        Q[i] = dalloc<cplx>(c, (size_t)rows * cols);
+        launch_eye_rect(Q[i], rows, cols, c->stream);
+        std::vector<QrApplyJob> app{QrApplyJob{&jobs[0], Q[i], cols}};
+        run_qr_apply_q(c, app);
+        RH[i] = dalloc<cplx>(c, (size_t)cols * cols);
+        std::vector<UpperHItem> uh{UpperHItem{M, rows, cols, RH[i]}};
+        std::vector<int2> ut;
+        for (size_t b = 0; b * 256 < (size_t)cols * cols; ++b) ut.push_back(make_int2(0, (int)b));
+        launch_upper_h_batch(upload(c, uh), upload(c, ut), (int)ut.size(), c->stream);
+        // M_{i+1} = R_i B_{i+1}  (χ_i × d χ_{i+1}); R_i = (R_i^H)^H
+        const int n2 = d * m->chi[i + 1];
+        cplx* nxt = dalloc<cplx>(c, (size_t)cols * n2);
+        std::vector<GemmItem> g{gemm_item(RH[i], GEMM_C, cols, m->site[i], GEMM_N, cols, nxt, cols, cols, n2, cols)};
+        run_gemm(c, g);
+        dfree(c, M);
+        dfree(c, v);
+        dfree(c, t);
+        M = nxt;
+    }
+    // SVDs of every bond matrix at once: the logical matrix is R_i^H = V s U^H, so the
+    // factor wanted, U_i, is its right factor
+    std::vector<SvdTask> tasks(n - 1);
+    std::vector<cplx*> U(n + 1, nullptr);
+    std::vector<double*> lam(n + 1, nullptr);
+    for (int i = 1; i <= n - 1; ++i) {
+        const int k = m->chi[i];
+        U[i] = dalloc<cplx>(c, (size_t)k * k);
+        lam[i] = dalloc<double>(c, k);
+        SvdTask& t = tasks[i - 1];
+        t.x = RH[i];
+        t.mr = t.mc = k;
+        t.truncate = 0;
+        t.refine = 1;
+        t.lam_dst = lam[i];
+        SvdOut o;
+        o.factor = 1;
+        o.dst = U[i];
+        t.outs = {o};
+    }
+    run_svd_tasks(c, tasks);
+    // Γ_i = pinv(s_{i-1}) U_{i-1}^H (Q_i U_i), batched over the sites
+    std::vector<double*> pinv(n + 1, nullptr);
+    for (int i = 1; i <= n - 1; ++i) {
+        pinv[i] = dalloc<double>(c, m->chi[i]);
+        launch_pinv(lam[i], m->chi[i], 1e-64, pinv[i], c->stream);
+    }
+    std::vector<cplx*> QU(n + 1, nullptr), G(n + 1, nullptr);
+    std::vector<GemmItem> g1, g2;
+    for (int i = 1; i <= n - 1; ++i) {
+        const int rows = m->chi[i - 1] * d, k = m->chi[i];
+        QU[i] = dalloc<cplx>(c, (size_t)rows * k);
+        g1.push_back(gemm_item(Q[i], GEMM_N, rows, U[i], GEMM_N, k, QU[i], rows, rows, k, k));
+    }
+    run_gemm(c, g1);
+    G[1] = QU[1];
+    for (int i = 2; i <= n; ++i) {
+        const int kl = m->chi[i - 1];
+        const int cols = d * m->chi[i];
+        const cplx* src = (i == n) ? M : QU[i];  // (χ_{i-1} × d χ_i)
+        G[i] = dalloc<cplx>(c, (size_t)kl * cols);
+        GemmItem g = gemm_item(U[i - 1], GEMM_C, kl, src, GEMM_N, kl, G[i], kl, kl, cols, kl);
+        g.rowscale = pinv[i - 1];
+        g2.push_back(g);
+    }
+    run_gemm(c, g2);
+    for (int i = 1; i <= n; ++i) replace_site(m, i, G[i]);
+    for (int i = 1; i <= n - 1; ++i) replace_lam(m, i, lam[i]);
+    for (int i = 2; i <= n - 1; ++i) dfree(c, QU[i]);
+    for (int i = 1; i <= n - 1; ++i) {
+        dfree(c, Q[i]);
+        dfree(c, RH[i]);
+        dfree(c, U[i]);
+        dfree(c, pinv[i]);
+    }
+    dfree(c, M);
     ctx_sync(c);
 }
 
@@ -690,6 +803,210 @@ std::vector<cplx*> left_envs(qh_context* c, const qh_mps* ket, const qh_mps* bra
     return L;
 }
 
+
+// Left environments L_i (bond i, without Λ_i) and right environments R_i of one open
+// chain (bra = ket), both sweeps advanced in lockstep: each launch carries the left
+// sweep's step at site k and the right sweep's at site n + 1 - k (the GEMMs of
+// env_segment and right_envs, so the values are the same).
+void lr_envs(qh_context* c, const qh_mps* m, std::vector<cplx*>& L, std::vector<cplx*>& R) {
+    const int n = m->n, d = m->d;
+    L.assign(n + 1, nullptr);
+    R.assign(n + 1, nullptr);
+    L[0] = unit_env(c);
+    R[n] = unit_env(c);
+    for (int k = 1; k <= n; ++k) {
+        const int i = k, j = n + 1 - k;  // left site, right site
+        const int la = m->chi[i - 1], ra = m->chi[i];
+        cplx* TL = dalloc<cplx>(c, (size_t)la * d * ra);
+        const int ja = m->chi[j - 1], jr = m->chi[j];
+        cplx* TR = dalloc<cplx>(c, (size_t)ja * d * jr);
+        std::vector<GemmItem> g1{gemm_item(L[i - 1], GEMM_N, la, m->site[i - 1], GEMM_N, la, TL, la, la, d * ra, la),
+                                 gemm_item(m->site[j - 1], GEMM_N, ja * d, R[j], GEMM_N, jr, TR, ja * d, ja * d, jr,
+                                           jr)};
+        if (i >= 2) { g1[0].dk = m->lam[i - 2]; g1[0].dk_mod = la; }
+        if (j <= n - 1) { g1[1].dk = m->lam[j - 1]; g1[1].dk_mod = jr; }
+        run_gemm(c, g1);
+        L[i] = dalloc<cplx>(c, (size_t)ra * ra);
+        R[j - 1] = dalloc<cplx>(c, (size_t)ja * ja);
+        std::vector<GemmItem> g2{gemm_item(m->site[i - 1], GEMM_C, la * d, TL, GEMM_N, la * d, L[i], ra, ra, ra, la * d),
+                                 gemm_item(TR, GEMM_N, ja, m->site[j - 1], GEMM_C, ja, R[j - 1], ja, ja, ja, d * jr)};
+        if (i >= 2) { g2[0].dk = m->lam[i - 2]; g2[0].dk_mod = la; }
+        if (j <= n - 1) { g2[1].dk = m->lam[j - 1]; g2[1].dk_mod = jr; g2[1].dk_div = d; }
+        run_gemm(c, g2);
+        dfree(c, TL);
+        dfree(c, TR);
+    }
+}
+
+// All terms of expect_many on one open chain, their environment segments advanced
+// together.  A term is a sequence of site steps from its first to its last site
+// (site i, and the cluster of operators starting there if any, as in env_segment);
+// the terms form a prefix tree over (first site, steps), so terms that begin alike
+// share their common steps.  The tree is processed depth by depth; within a depth
+// the micro-operations of all nodes (absorb a ket site, apply the cluster, peel a
+// bra site) go out as one batched GEMM or one batched operator launch per phase.
+// vals[t] receives term t's value.
+void expect_terms_batched(qh_context* c, const qh_mps* m, const std::vector<std::vector<Cluster>>& terms,
+                          cplx* vals) {
+    const int n = m->n, d = m->d;
+    struct Node {
+        int parent = -1, depth = 0;
+        int start = 0;            // roots: first site of their terms
+        int site = 0, width = 1;  // the site step (non-roots)
+        const Cluster* cl = nullptr;
+        cplx* E = nullptr;        // environment after the step (roots: L[start - 1], borrowed)
+        bool leaf = false;
+        int children = 0;
+    };
+    std::vector<Node> nodes;
+    std::map<std::pair<int, std::string>, int> index;  // (parent or -start-1, step key) -> node
+    std::vector<int> term_leaf(terms.size());
+    std::vector<int> term_end(terms.size());
+    int maxdepth = 0;
+    for (size_t t = 0; t < terms.size(); ++t) {
+        const auto& cls = terms[t];
+        int s = 1, e = 1;
+        if (!cls.empty()) {
+            s = cls.front().start;
+            e = cls.back().start + cls.back().width - 1;
+        }
+        term_end[t] = e;
+        auto rk = std::make_pair(-s - 1, std::string());
+        int cur;
+        if (!index.count(rk)) {
+            Node r;
+            r.start = s;
+            cur = (int)nodes.size();
+            index[rk] = cur;
+            nodes.push_back(r);
+        } else {
+            cur = index[rk];
+        }
+        size_t ci = 0;
+        int i = s;
+        while (i <= e) {
+            const bool has = ci < cls.size() && cls[ci].start == i;
+            const int w = has ? cls[ci].width : 1;
+            std::string key = std::to_string(i) + ":" + std::to_string(w);
+            if (has) key.append(reinterpret_cast<const char*>(cls[ci].op.data()), cls[ci].op.size() * sizeof(zc));
+            auto k2 = std::make_pair(cur, key);
+            auto it = index.find(k2);
+            if (it == index.end()) {
+                Node nd;
+                nd.parent = cur;
+                nd.depth = nodes[cur].depth + 1;
+                nd.site = i;
+                nd.width = w;
+                nd.cl = has ? &cls[ci] : nullptr;
+                nodes[cur].children += 1;
+                const int id = (int)nodes.size();
+                index[k2] = id;
+                nodes.push_back(nd);
+                cur = id;
+                maxdepth = std::max(maxdepth, nd.depth);
+            } else {
+                cur = it->second;
+            }
+            i += w;
+            if (has) ++ci;
+        }
+        nodes[cur].leaf = true;
+        term_leaf[t] = cur;
+    }
+    std::vector<cplx*> L, R;
+    lr_envs(c, m, L, R);
+    for (auto& nd : nodes)
+        if (nd.parent < 0) nd.E = L[nd.start - 1];
+    std::vector<std::vector<int>> by_depth(maxdepth + 1);
+    for (int k = 0; k < (int)nodes.size(); ++k) by_depth[nodes[k].depth].push_back(k);
+    // operators of the clusters in device memory (one upload per distinct node)
+    std::vector<const cplx*> dop(nodes.size(), nullptr);
+    for (int k = 0; k < (int)nodes.size(); ++k)
+        if (nodes[k].cl) {
+            std::vector<cplx> h(nodes[k].cl->op.size());
+            std::memcpy(h.data(), nodes[k].cl->op.data(), h.size() * sizeof(cplx));
+            dop[k] = upload(c, h);
+        }
+    for (int depth = 1; depth <= maxdepth; ++depth) {
+        const auto& lvl = by_depth[depth];
+        // per node: the current buffer and its micro-operation counter
+        std::vector<cplx*> cur(lvl.size());
+        std::vector<int> curl(lvl.size());
+        for (size_t q = 0; q < lvl.size(); ++q) {
+            cur[q] = nodes[nodes[lvl[q]].parent].E;
+            curl[q] = m->chi[nodes[lvl[q]].site - 1];
+        }
+        int maxw = 1;
+        for (int k : lvl) maxw = std::max(maxw, nodes[k].width);
+        const int phases = 2 * maxw + 1;  // w ket absorptions, the operator, w bra peels
+        for (int ph = 0; ph < phases; ++ph) {
+            std::vector<GemmItem> gm;
+            std::vector<cplx*> to_free;
+            std::vector<ApplyItem> ap;
+            std::vector<int2> ap_tiles;
+            for (size_t q = 0; q < lvl.size(); ++q) {
+                const Node& nd = nodes[lvl[q]];
+                const int i = nd.site, w = nd.width;
+                const int lb = m->chi[i - 1], ra = m->chi[i + w - 1];
+                // this node's phase ph: ket absorption k < w, then the operator, then bra peels
+                const int nph = 2 * w + 1;
+                if (ph >= nph) continue;
+                if (ph < w) {
+                    const int j = i + ph;
+                    const int rows = lb * ipow(d, ph);
+                    cplx* T = dalloc<cplx>(c, (size_t)rows * d * m->chi[j]);
+                    GemmItem g = gemm_item(cur[q], GEMM_N, rows, m->site[j - 1], GEMM_N, m->chi[j - 1], T, rows, rows,
+                                           d * m->chi[j], m->chi[j - 1]);
+                    if (j >= 2) { g.dk = m->lam[j - 2]; g.dk_mod = m->chi[j - 1]; }
+                    gm.push_back(g);
+                    if (ph > 0) to_free.push_back(cur[q]);
+                    cur[q] = T;
+                } else if (ph == w) {
+                    if (nd.cl) {
+                        const size_t pairs = (size_t)lb * ra;
+                        for (size_t b = 0; b * 256 < pairs; ++b) ap_tiles.push_back(make_int2((int)ap.size(), (int)b));
+                        ap.push_back(ApplyItem{cur[q], lb, ipow(d, w), ra, dop[lvl[q]]});
+                    }
+                } else {
+                    const int s2 = ph - w - 1;
+                    const int j = i + s2;
+                    const int rest = ipow(d, w - 1 - s2) * ra;
+                    const int rb = m->chi[j];
+                    cplx* W = dalloc<cplx>(c, (size_t)rb * rest);
+                    GemmItem g = gemm_item(m->site[j - 1], GEMM_C, curl[q] * d, cur[q], GEMM_N, curl[q] * d, W, rb,
+                                           rb, rest, curl[q] * d);
+                    if (j >= 2) { g.dk = m->lam[j - 2]; g.dk_mod = curl[q]; }
+                    gm.push_back(g);
+                    to_free.push_back(cur[q]);
+                    cur[q] = W;
+                    curl[q] = rb;
+                }
+            }
+            if (!ap.empty()) launch_apply_op_batch(upload(c, ap), upload(c, ap_tiles), (int)ap_tiles.size(), c->stream);
+            run_gemm(c, gm);
+            for (cplx* p : to_free) dfree(c, p);
+        }
+        for (size_t q = 0; q < lvl.size(); ++q) nodes[lvl[q]].E = cur[q];
+        // environments of the previous depth are no longer read, unless a term ends there
+        for (int k : by_depth[depth - 1])
+            if (nodes[k].parent >= 0 && !nodes[k].leaf) {
+                dfree(c, nodes[k].E);
+                nodes[k].E = nullptr;
+            }
+    }
+    std::vector<CloseItem> close(terms.size());
+    for (size_t t = 0; t < terms.size(); ++t) {
+        const int e = term_end[t];
+        const double* lb = e <= n - 1 ? m->lam[e - 1] : nullptr;
+        close[t] = CloseItem{nodes[term_leaf[t]].E, R[e], m->chi[e], m->chi[e], lb, lb, vals + t};
+    }
+    launch_env_close_batch(upload(c, close), (int)close.size(), c->stream);
+    for (auto& nd : nodes)
+        if (nd.parent >= 0 && nd.E) dfree(c, nd.E);
+    for (auto p : L) dfree(c, p);
+    for (auto p : R) dfree(c, p);
+}
+
 }  // namespace
 
 namespace {
@@ -773,6 +1090,7 @@ int qh_ctx_create(int device, qh_context** out) {
         c = new qh_context();
         c->device = device;
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (const char* e = std::getenv("QH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         // a small first chunk makes tests exercise the multi-chunk arena cheaply
         if (const char* e = std::getenv("QH_ARENA_INITIAL_BYTES")) {
             const long long v = std::atoll(e);
@@ -849,7 +1167,10 @@ int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
     if (!c) return QH_ERR_ARGUMENT;
     return guard(c, [&] {
         if (block != 0 && block != 8 && block != 16) throw Error(QH_ERR_ARGUMENT, "Jacobi block must be 8 or 16");
-        if (block) c->jac_block = block;
+        if (block) {
+            c->jac_block = block;
+            c->tuned_block.clear();  // an explicit setting replaces the tuned one
+        }
         if (max_sweeps > 0) c->jac_max_sweeps = max_sweeps;
         if (const char* e = std::getenv("QH_JACOBI_MAX_INNER")) c->jac_max_inner = std::max(1, std::atoi(e));
     });
@@ -860,6 +1181,99 @@ int qh_ctx_set_qr_passes(qh_context* c, int passes) {
     return guard(c, [&] {
         if (passes < 0 || passes > QR_MAX_PASSES) throw Error(QH_ERR_ARGUMENT, "QR passes must be in 0..4");
         c->jac_precondition = passes;
+        c->tuned_passes.clear();  // an explicit setting replaces the tuned one
+    });
+}
+
+// The SVD schedule tuner.  For each size class q = 64, 128, ... <= max_q it factors a
+// batch of q × q synthetic matrices -- half with i.i.d. uniform entries (a flat
+// spectrum, like Haar-random gates on full-rank states), half with their columns
+// scaled from 1 down to 1e-12 (a decaying spectrum, like a truncated time evolution)
+// -- through the batched SVD under each candidate schedule (Jacobi block width 8 or
+// 16, 1..4 QR preconditioning passes), timed end to end after one warm-up run, and
+// keeps the fastest for the class.  Batches shrink with q² above q = 512 so every
+// class costs about the same.  The candidates change the rounding of the results,
+// never their meaning.
+int qh_ctx_autotune(qh_context* c, int max_q, int batch, double* report, int* nclasses) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        if (max_q < 64 || max_q > 4096) throw Error(QH_ERR_ARGUMENT, "max_q must be in 64..4096");
+        if (batch < 2 || batch > 512) throw Error(QH_ERR_ARGUMENT, "batch must be in 2..512");
+        const int keep_block = c->jac_block, keep_passes = c->jac_precondition;
+        std::map<int, int> passes_tab, block_tab;
+        c->tuned_passes.clear();
+        c->tuned_block.clear();
+        int ncls = 0;
+        for (int q = 64; q <= max_q; q *= 2, ++ncls) {
+            const int b = std::max(2, q <= 512 ? batch : (int)((long long)batch * 512 * 512 / ((long long)q * q)));
+            const size_t elems = (size_t)q * q;
+            std::vector<cplx*> src(b), work(b);
+            std::vector<double*> lam(b);
+            for (int i = 0; i < b; ++i) {
+                src[i] = dalloc<cplx>(c, elems);
+                work[i] = dalloc<cplx>(c, elems);
+                lam[i] = dalloc<double>(c, q);
+                launch_rand_uniform(src[i], elems, 0x5eedull + (unsigned long long)q, (unsigned long long)i, 1,
+                                    c->stream);
+                if (i % 2) launch_scale_cols_geometric(src[i], q, q, 12.0, c->stream);
+            }
+            auto run = [&](int block, int passes) {
+                c->jac_block = block;
+                c->jac_precondition = passes;
+                for (int i = 0; i < b; ++i)
+                    QH_HIP(hipMemcpyAsync(work[i], src[i], elems * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+                std::vector<SvdTask> tasks(b);
+                for (int i = 0; i < b; ++i) {
+                    tasks[i].x = work[i];
+                    tasks[i].mr = tasks[i].mc = q;
+                    tasks[i].truncate = 0;
+                    tasks[i].lam_dst = lam[i];
+                }
+                ctx_sync(c);
+                const auto t0 = std::chrono::steady_clock::now();
+                run_svd_tasks(c, tasks);  // synchronises
+                return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            };
+            (void)run(16, 3);  // warm-up: allocations of the pool, first launches
+            double best = 1e300, dflt = 0.0;
+            int bb = 16, bp = 3;
+            for (int block : {8, 16})
+                for (int passes = 1; passes <= QR_MAX_PASSES; ++passes) {
+                    const double ms = run(block, passes);
+                    if (block == keep_block && passes == keep_passes) dflt = ms;
+                    if (ms < best) { best = ms; bb = block; bp = passes; }
+                }
+            passes_tab[q] = bp;
+            block_tab[q] = bb;
+            if (report && ncls < 8) {
+                double* r = report + 5 * ncls;
+                r[0] = q; r[1] = bb; r[2] = bp; r[3] = best; r[4] = dflt;
+            }
+            for (int i = 0; i < b; ++i) {
+                dfree(c, src[i]);
+                dfree(c, work[i]);
+                dfree(c, lam[i]);
+            }
+        }
+        c->jac_block = keep_block;
+        c->jac_precondition = keep_passes;
+        c->tuned_passes = passes_tab;
+        c->tuned_block = block_tab;
+        if (nclasses) *nclasses = std::min(ncls, 8);
+    });
+}
+
+int qh_ctx_set_graphs(qh_context* c, int enable) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { c->use_graphs = enable != 0; });
+}
+
+// forget the tuned schedule (back to qh_ctx_set_jacobi / qh_ctx_set_qr_passes)
+int qh_ctx_autotune_reset(qh_context* c) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        c->tuned_passes.clear();
+        c->tuned_block.clear();
     });
 }
 
@@ -875,6 +1289,67 @@ int qh_mps_create(qh_context* c, int n, int d, const int* bond_dims, const doubl
 
 int qh_pmps_create(qh_context* c, int n, int d, const int* bond_dims, const double* const* arrays, qh_mps** out) {
     return mps_create_impl(c, n, d, bond_dims, arrays, nullptr, true, out);
+}
+
+// `rand(Chain, Open, State; n, χ, p, eltype)` (src/Ansatz/Chain.jl:223-256): bond b
+// has dimension min(χ, p^b, p^(n-b)); every site but the first is a right isometry
+// (orthonormal rows of a matrix of uniform [0, 1) entries, `gramschmidt!` at
+// Chain.jl:242), the first is p × χ1 with orthonormal columns divided by sqrt(p), so
+// the state has norm 1 when χ >= p.  The orthonormalisation is the batched
+// Householder QR of the transposed random matrices (its phases differ from
+// Gram-Schmidt's, a gauge), the entries a counter-based hash of (seed, site, entry).
+int qh_mps_rand(qh_context* c, int n, int chi, int p, int complex_entries, unsigned long long seed, qh_mps** out) {
+    if (!c || !out) return QH_ERR_ARGUMENT;
+    return guard(c, [&] {
+        if (n < 2) throw Error(QH_ERR_ARGUMENT, "a Chain needs at least 2 sites");
+        if (p < 2 || p > 4) throw Error(QH_ERR_ARGUMENT, "physical dimension must be 2, 3 or 4");
+        if (chi < 1) throw Error(QH_ERR_ARGUMENT, "χ must be positive");
+        std::unique_ptr<qh_mps> m(new qh_mps());
+        m->ctx = c;
+        m->n = n;
+        m->d = p;
+        m->chi.assign(n + 1, 1);
+        for (int b = 1; b <= n - 1; ++b) {
+            long long v = 1;
+            for (int e = 0; e < std::min(b, n - b) && v < chi; ++e) v *= p;
+            m->chi[b] = (int)std::min<long long>(v, chi);
+        }
+        m->site.assign(n, nullptr);
+        m->lam.assign(n - 1, nullptr);
+        // site 1: X (p × χ1); site i >= 2: X = S^H ((p χ_i) × χ_{i-1}), S the site as χl × (p χr)
+        std::vector<QrJob> jobs(n);
+        std::vector<cplx*> q(n);
+        for (int i = 1; i <= n; ++i) {
+            const int rows = (i == 1) ? p : p * m->chi[i];
+            const int cols = (i == 1) ? m->chi[1] : m->chi[i - 1];
+            cplx* x = dalloc<cplx>(c, (size_t)rows * cols);
+            launch_rand_uniform(x, (size_t)rows * cols, seed, (unsigned long long)i, complex_entries, c->stream);
+            const size_t tq = (size_t)((cols + QR_NB - 1) / QR_NB) * QR_NB * QR_NB;
+            jobs[i - 1] = QrJob{x, rows, cols, rows, dalloc<cplx>(c, (size_t)rows * cols), dalloc<cplx>(c, tq)};
+            q[i - 1] = dalloc<cplx>(c, (size_t)rows * cols);
+            launch_eye_rect(q[i - 1], rows, cols, c->stream);
+        }
+        run_qr_factor(c, jobs);
+        std::vector<QrApplyJob> app;
+        for (int i = 1; i <= n; ++i) app.push_back(QrApplyJob{&jobs[i - 1], q[i - 1], jobs[i - 1].n});
+        run_qr_apply_q(c, app);
+        for (int i = 1; i <= n; ++i) {
+            const QrJob& j = jobs[i - 1];
+            if (i == 1) {
+                launch_scale_real(q[0], (size_t)j.m * j.n, 1.0 / std::sqrt((double)p), c->stream);
+                m->site[0] = q[0];  // (1, o, r) = Q[o, r] / sqrt(p)
+            } else {
+                m->site[i - 1] = dalloc<cplx>(c, (size_t)j.m * j.n);
+                launch_conj_transpose(q[i - 1], j.m, j.n, m->site[i - 1], c->stream);  // S = Q^H
+                dfree(c, q[i - 1]);
+            }
+            dfree(c, j.a);
+            dfree(c, j.v);
+            dfree(c, j.t);
+        }
+        for (auto s2 : m->site) keep(c, s2);
+        *out = m.release();
+    });
 }
 
 int qh_mps_is_periodic(const qh_mps* m, int* periodic, int* wrap_dim) {
@@ -1100,27 +1575,12 @@ int qh_expect_many(qh_context* c, qh_mps* m, int nterms, const int* offsets, con
             }
             return;
         }
-        auto L = left_envs(c, m, m);
-        auto R = right_envs(c, m, m);
         cplx* vals = dalloc<cplx>(c, std::max(nterms, 1));
-        for (int t = 0; t < nterms; ++t) {
-            int s = 1, e = 1;
-            if (!terms[t].empty()) {
-                s = terms[t].front().start;
-                e = terms[t].back().start + terms[t].back().width - 1;
-            }
-            cplx* E = dcopy(c, L[s - 1], (size_t)m->chi[s - 1] * m->chi[s - 1]);
-            E = env_segment(c, m, m, E, s, e, terms[t]);
-            const double* lb = e <= m->n - 1 ? m->lam[e - 1] : nullptr;
-            launch_env_close(E, R[e], m->chi[e], m->chi[e], lb, lb, vals + t, c->stream);
-            dfree(c, E);
-        }
+        expect_terms_batched(c, m, terms, vals);
         std::vector<cplx> hv(std::max(nterms, 1));
         readback(c, hv.data(), vals, hv.size() * sizeof(cplx));
         std::memcpy(out, hv.data(), (size_t)nterms * sizeof(cplx));
         dfree(c, vals);
-        for (auto p : L) dfree(c, p);
-        for (auto p : R) dfree(c, p);
     });
 }
 

@@ -99,12 +99,15 @@ void launch_frob2(const JacItem* d_items, int count, const int2* d_tiles, int nt
                   const int* first, hipStream_t s);
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
-void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, int block,
-                         int sort_sweeps, int debug, int max_inner, hipStream_t s);
+// round: the tournament round, plus *round_base when round_base is not null
+void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
+                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // converged <- (no rotation in the window); *remaining = items still rotating
-void launch_conv_update(const JacItem* d_items, int count, int* remaining, hipStream_t s);
+// and, if round_base is not null, *round_base += rounds (the next sweep's first round)
+void launch_conv_update(const JacItem* d_items, int count, int* remaining, int* round_base, int rounds,
+                        hipStream_t s);
 
 // Finalisation: σ_j = ||X[:, j]||, sort descending, truncation rule of
 // `truncate!`, optional renormalisation, kept count and discarded weight.
@@ -144,6 +147,20 @@ struct CompleteItem {
     double floor_rel2;
 };
 void launch_complete(const CompleteItem* d_items, int count, hipStream_t s);
+
+// x[r, perm[j]] = f[r, j] (times σ_j if mul_sigma and σ_j > 0) for j < kept: the
+// inverse of a gather (tile map: item, column j)
+struct ScatterItem {
+    cplx* x;
+    int p;
+    const int* perm;
+    const double* sigma_sorted;
+    const int* kept;
+    const cplx* f;
+    int ldf;
+    int mul_sigma;
+};
+void launch_scatter_cols(const ScatterItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 
 // out[r, j] = src[r, perm[j]] * colfac(j) * rowscale[(r / div) % mod] for j < kept
 //   colfac = 1/sigma_sorted[j] if divide_sigma (1 if sigma == 0: see CompleteItem) else 1
@@ -228,6 +245,31 @@ void launch_gram_dev(const cplx* gram, int n, double* dev_out, hipStream_t s);
 void launch_env_close(const cplx* e, const cplx* r, int nb, int na, const double* lb, const double* la, cplx* out,
                       hipStream_t s);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
+// a local operator on the middle index of many tensors T[l, μ, r] (dim <= 16)
+struct ApplyItem {
+    cplx* t;
+    int left, dim, right;
+    const cplx* op;
+};
+// tile map: (item, block of 256 (l, r) pairs)
+void launch_apply_op_batch(const ApplyItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// closing dots Σ_{b,a} E[b, a] R[a, b] λb[b] λa[a] of many environments (one workgroup each)
+struct CloseItem {
+    const cplx* e;
+    const cplx* r;
+    int nb, na;
+    const double* lb;
+    const double* la;
+    cplx* out;
+};
+void launch_env_close_batch(const CloseItem* d_items, int count, hipStream_t s);
+// p[e] <- uniform [0, 1) (+ i·uniform [0, 1) if complex_entries), a hash of (seed, key, e)
+void launch_rand_uniform(cplx* p, size_t n, unsigned long long seed, unsigned long long key, int complex_entries,
+                         hipStream_t s);
+// p[e] <- f · p[e]
+void launch_scale_real(cplx* p, size_t n, double f, hipStream_t s);
+// column j of a rows × cols matrix times 10^(-decades · j / (cols - 1))
+void launch_scale_cols_geometric(cplx* p, int rows, int cols, double decades, hipStream_t s);
 void launch_eye(cplx* p, int n, hipStream_t s);
 
 }  // namespace qh

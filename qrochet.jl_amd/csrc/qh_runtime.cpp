@@ -316,14 +316,14 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
 // ---------------------------------------------------------------------------
 // Jacobi SVD
 // ---------------------------------------------------------------------------
-void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out) {
+void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, int block) {
     if (items.empty()) return;
-    const int block = c->jac_block;
+    if (block <= 0) block = c->jac_block;
     const int bc = 2 * block;
     std::vector<int2> id_tiles, tiles;
     int rmax = 1;
-    int* counters = dalloc<int>(c, 2 * items.size() + 2);
-    QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 2) * sizeof(int), c->stream));
+    int* counters = dalloc<int>(c, 2 * items.size() + 3);
+    QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 3) * sizeof(int), c->stream));
     double* fro2 = fro2_out ? fro2_out : dalloc<double>(c, items.size());
     auto free_fro2 = [&] {
         if (!fro2_out) dfree(c, fro2);
@@ -377,18 +377,51 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out) {
     std::vector<char> converged(items.size(), 0);
     std::vector<int> hrot(2 * items.size());
     bool done = false;
-    for (int sweep = 0; sweep < c->jac_max_sweeps; ++sweep) {
+    // Items still rotating at the sweep cap get a second window of the same length
+    // before the call fails; the converged ones are skipped by every launch, so the
+    // extra sweeps cost only the stragglers' work (and the launches' latency).
+    const int hard_cap = 2 * c->jac_max_sweeps;
+    // A sweep's rounds as one HIP graph, captured at the first sweep and replayed by
+    // every later one: the rounds read their index as an offset from a counter in
+    // device memory that k_conv_update advances by the sweep's length, so each replay
+    // runs the next sweep's rounds (the same indices as direct launches).
+    static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
+    const bool use_graph = c->use_graphs && !kdbg;
+    int* round_base = use_graph ? remaining + 2 : nullptr;
+    struct SweepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~SweepGraph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    } sg;
+    for (int sweep = 0; sweep < hard_cap; ++sweep) {
         // read the per-item rotation counters after the window to account work
         {
             // one event pair per window of back-to-back rounds (per-launch events would
             // cost more host time than the short rounds themselves)
             Timed tm(c, "jacobi_round");
             if (c->timing) c->timers["jacobi_round"].launches += rmax - 1;
-            static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
-            for (int r = 0; r < rmax; ++r) {
-                const int g = sweep * rmax + r;
-                launch_jacobi_round(di, dt, (int)tiles.size(), g, block, c->jac_sort_sweeps, kdbg, c->jac_max_inner,
-                                    c->stream);
+            if (use_graph) {
+                if (!sg.exec) {
+                    QH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                    for (int r = 0; r < rmax; ++r)
+                        launch_jacobi_round(di, dt, (int)tiles.size(), r, round_base, block, c->jac_sort_sweeps, 0,
+                                            c->jac_max_inner, c->stream);
+                    const hipError_t e1 = hipGetLastError();
+                    const hipError_t e2 = hipStreamEndCapture(c->stream, &sg.graph);
+                    QH_HIP(e1);
+                    QH_HIP(e2);
+                    QH_HIP(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
+                }
+                QH_HIP(hipGraphLaunch(sg.exec, c->stream));
+            } else {
+                for (int r = 0; r < rmax; ++r) {
+                    const int g = sweep * rmax + r;
+                    launch_jacobi_round(di, dt, (int)tiles.size(), g, nullptr, block, c->jac_sort_sweeps, kdbg,
+                                        c->jac_max_inner, c->stream);
+                }
             }
         }
         static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
@@ -411,7 +444,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out) {
                 if (hrot[2 * i] == 0) converged[i] = 1;
             }
         }
-        launch_conv_update(di, (int)items.size(), remaining, c->stream);
+        launch_conv_update(di, (int)items.size(), remaining, round_base, rmax, c->stream);
         c->timers["jacobi_sweep"].launches += 1;
         int rem[2] = {0, 0};
         readback(c, rem, remaining, sizeof(rem));
@@ -434,7 +467,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out) {
     if (!done) {
         std::vector<int> h(2 * items.size());
         readback(c, h.data(), counters, h.size() * sizeof(int));
-        std::string msg = "Jacobi SVD did not converge within " + std::to_string(c->jac_max_sweeps) + " sweeps:";
+        std::string msg = "Jacobi SVD did not converge within " + std::to_string(hard_cap) + " sweeps:";
         if (const char* dump = std::getenv("QH_DEBUG_JACOBI_DUMP")) {
             for (size_t i = 0; i < items.size(); ++i)
                 if (!h[2 * i + 1]) {
@@ -567,6 +600,46 @@ void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs) {
     launch_qr_form(da, dt, (int)tiles.size(), c->stream);
 }
 
+// F <- F (3I - F^H F) / 2 for every job (in place; zero columns stay zero)
+void polar_refine(qh_context* c, const std::vector<PolarJob>& jobs) {
+    if (jobs.empty()) return;
+    std::vector<cplx*> G(jobs.size()), F2(jobs.size());
+    std::vector<GemmItem> gg, gf;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        const PolarJob& j = jobs[q];
+        G[q] = dalloc<cplx>(c, (size_t)j.cols * j.cols);
+        F2[q] = dalloc<cplx>(c, (size_t)j.rows * j.cols);
+        gg.push_back(gemm_item(j.f, GEMM_C, j.rows, j.f, GEMM_N, j.rows, G[q], j.cols, j.cols, j.cols, j.rows));
+        GemmItem h = gemm_item(j.f, GEMM_N, j.rows, G[q], GEMM_N, j.cols, F2[q], j.rows, j.rows, j.cols, j.cols);
+        h.alpha = make_double2(-0.5, 0.0);
+        h.beta = make_double2(1.5, 0.0);
+        gf.push_back(h);
+    }
+    run_gemm(c, gg);
+    for (size_t q = 0; q < jobs.size(); ++q)
+        QH_HIP(hipMemcpyAsync(F2[q], jobs[q].f, (size_t)jobs[q].rows * jobs[q].cols * sizeof(cplx),
+                              hipMemcpyDeviceToDevice, c->stream));
+    run_gemm(c, gf);
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        QH_HIP(hipMemcpyAsync(jobs[q].f, F2[q], (size_t)jobs[q].rows * jobs[q].cols * sizeof(cplx),
+                              hipMemcpyDeviceToDevice, c->stream));
+        dfree(c, G[q]);
+        dfree(c, F2[q]);
+    }
+}
+
+// QR preconditioning passes for a matrix with q columns: the tuned value of its size
+// class (qh_ctx_autotune) if there is one, else the context's setting
+int qr_passes_for(const qh_context* c, int q) {
+    int passes = c->jac_precondition;
+    if (!c->tuned_passes.empty()) {
+        auto it = c->tuned_passes.lower_bound(q);  // the smallest class >= q
+        if (it == c->tuned_passes.end()) --it;
+        passes = it->second;
+    }
+    return std::min(std::max(passes, 0), QR_MAX_PASSES);
+}
+
 int SvdTask::kmax() const {
     const int k = std::min(mr, mc);
     if (truncate && maxdim >= 0) return std::min(k, maxdim);
@@ -576,7 +649,16 @@ int SvdTask::kmax() const {
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     if (tasks.empty()) return;
     const size_t nt = tasks.size();
-    const int npass = std::min(std::max(c->jac_precondition, 0), QR_MAX_PASSES);
+    // Jacobi block width of this batch: the tuned one of its largest size class, else
+    // the context's setting
+    int bw = c->jac_block;
+    if (!c->tuned_block.empty()) {
+        int qmax = 0;
+        for (const SvdTask& t : tasks) qmax = std::max(qmax, std::min(t.mr, t.mc));
+        auto it = c->tuned_block.lower_bound(qmax);
+        if (it == c->tuned_block.end()) --it;
+        bw = it->second;
+    }
     // workspace layout per task
     size_t bytes = 0;
     auto take = [&](size_t b) {
@@ -588,6 +670,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         size_t v, s, p, l, ux, vx, tmp;
         size_t qv[QR_MAX_PASSES + 1], qt[QR_MAX_PASSES + 1], y[QR_MAX_PASSES + 1];
         bool pre;
+        int np;  // QR preconditioning passes of this task (0 without preconditioning)
         int p_, q_, k_;
     };
     std::vector<Off> off(nt);
@@ -597,8 +680,10 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         o.p_ = std::max(t.mr, t.mc);
         o.q_ = std::min(t.mr, t.mc);
         o.k_ = t.kmax();
-        const size_t p = o.p_, q = o.q_, k = std::max(o.k_, 1);
-        o.pre = npass > 0 && (int)q > 2 * c->jac_block;
+        const size_t p = o.p_, q = o.q_;
+        o.np = (int)q > 2 * bw ? qr_passes_for(c, (int)q) : 0;
+        o.pre = o.np > 0;
+        const int npass = o.np;
         o.v = take(q * q * sizeof(cplx));
         o.s = take(q * sizeof(double));
         o.p = take(q * sizeof(int));
@@ -610,17 +695,20 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
                 o.qt[j] = take(tq * sizeof(cplx));
                 o.y[j] = take(q * q * sizeof(cplx));
             }
-            o.vx = take(q * k * sizeof(cplx));
+            o.vx = take(q * std::max(o.k_, 1) * sizeof(cplx));
         }
     }
-    // the buffers that start as zeros (ux and tmp) sit together, cleared by one fill
+    // The buffers that start as zeros (ux and tmp) sit together, cleared by one fill.
+    // (This placement of the per-task buffers is the measured one: with vx moved into
+    // the zero-filled run, the same Jacobi rounds took 9 % longer on MI355X -- the X and
+    // V buffers of the batch's matrices then sit at other strides in HBM.)
     const size_t zero_begin = bytes;
     for (size_t i = 0; i < nt; ++i) {
         Off& o = off[i];
         if (!o.pre) continue;
         const size_t p = o.p_, q = o.q_, k = std::max(o.k_, 1);
         o.ux = take(p * k * sizeof(cplx));
-        if (npass >= 3) o.tmp = take(q * k * sizeof(cplx));
+        if (o.np >= 3) o.tmp = take(q * k * sizeof(cplx));
     }
     const size_t zero_end = bytes;
     const size_t off_k = take(nt * sizeof(int));
@@ -634,11 +722,13 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     // ---- QR preconditioning: X = Q1 R1 and R_{j-1}^H = Q_j R_j for j = 2..npass;
     // the Jacobi runs on Y = R_npass^H (lower triangular), y[j] = R_j^H
     std::vector<std::array<QrJob, QR_MAX_PASSES + 1>> qr(nt);
-    for (int j = 1; j <= npass; ++j) {
+    int max_np = 0;
+    for (const Off& o : off) max_np = std::max(max_np, o.np);
+    for (int j = 1; j <= max_np; ++j) {
         std::vector<QrJob> f;
         for (size_t i = 0; i < nt; ++i) {
             const Off& o = off[i];
-            if (!o.pre) continue;
+            if (o.np < j) continue;
             qr[i][j] = (j == 1) ? QrJob{tasks[i].x, o.p_, o.q_, o.p_, P(o.qv[1]), P(o.qt[1])}
                                 : QrJob{P(o.y[j - 1]), o.q_, o.q_, o.q_, P(o.qv[j]), P(o.qt[j])};
             f.push_back(qr[i][j]);
@@ -648,7 +738,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         std::vector<UpperHItem> uh;
         std::vector<int2> ut;
         for (size_t i = 0; i < nt; ++i)
-            if (off[i].pre) {
+            if (off[i].np >= j) {
                 const int q = off[i].q_;
                 for (size_t b = 0; b * 256 < (size_t)q * q; ++b) ut.push_back(make_int2((int)uh.size(), (int)b));
                 uh.push_back(UpperHItem{qr[i][j].a, qr[i][j].m, q, P(off[i].y[j])});
@@ -662,7 +752,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         const SvdTask& t = tasks[i];
         const Off& o = off[i];
         jac[i] = JacItem{};
-        jac[i].x = o.pre ? P(o.y[npass]) : t.x;
+        jac[i].x = o.pre ? P(o.y[o.np]) : t.x;
         jac[i].v = P(o.v);
         jac[i].p = o.pre ? o.q_ : o.p_;
         jac[i].q = o.q_;
@@ -682,7 +772,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         s.renormalize = t.renormalize;
     }
     double* d_fro2 = reinterpret_cast<double*>(ws + off_f);
-    run_jacobi(c, jac, d_fro2);
+    run_jacobi(c, jac, d_fro2, bw);
     run_select(c, sel);
     // kept columns the Jacobi left frozen (at or below its deflation floor) get an
     // orthonormal completion, as LAPACK's factors are orthonormal however small σ is
@@ -693,32 +783,72 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
                                    sel[i].kept, d_fro2 + i, jac[i].floor_rel2};
         launch_complete(upload(c, comp), (int)nt, c->stream);
     }
-
-    // ---- preconditioned tasks.  The Jacobi gives Y V_J = W with columns of norm σ;
-    // with Wn = W / σ and R_{j-1} = R_j^H Q_j^H unrolled:
-    //   npass even:  X = (Q1 Q3 .. Q_{n-1}) Y (Q2 Q4 .. Q_n)^H  ->  U_X = Q1 Q3 .. Wn,   V_X = Q2 Q4 .. V_J
-    //   npass odd:   X = (Q1 Q3 .. Q_n) Y^H (Q2 Q4 .. Q_{n-1})^H ->  U_X = Q1 Q3 .. V_J,  V_X = Q2 Q4 .. Wn
-    // (columns in sorted order; the factor next to the source is applied first).
-    const bool even = npass % 2 == 0;
-    std::vector<int> uq, vq;  // factors to apply on each side, innermost first
-    for (int j = npass; j >= 1; --j) (j % 2 ? uq : vq).push_back(j);
+    // polar refinement where asked (see SvdTask::refine).  Without preconditioning the
+    // factors are the kept columns of X (divided by σ) and of V, refined in dense
+    // copies and written back; with it, ux and vx below, after the Q factors.
+    {
+        std::vector<PolarJob> jobs;
+        std::vector<cplx*> tmp;
+        std::vector<GatherItem> gi;
+        std::vector<ScatterItem> si;
+        std::vector<int2> st;
+        for (size_t i = 0; i < nt; ++i) {
+            if (!tasks[i].refine || off[i].pre) continue;
+            const int k = std::max(off[i].k_, 1);
+            for (int side = 0; side < 2; ++side) {
+                const int rows = side == 0 ? jac[i].p : jac[i].q;
+                cplx* f = dalloc<cplx>(c, (size_t)rows * k);
+                tmp.push_back(f);
+                launch_fill(reinterpret_cast<double*>(f), 2 * (size_t)rows * k, 0.0, c->stream);
+                GatherItem g{};
+                g.src = side == 0 ? jac[i].x : jac[i].v; g.ld_src = rows; g.rows = rows; g.perm = sel[i].perm;
+                g.sigma_sorted = sel[i].sigma_sorted; g.divide_sigma = side == 0 ? 1 : 0; g.kept = sel[i].kept;
+                g.rowscale_mod = 1; g.rowscale_div = 1; g.out = f; g.ld_out = rows; g.max_cols = k;
+                gi.push_back(g);
+                jobs.push_back(PolarJob{f, rows, k});
+                for (int j = 0; j < k; ++j) st.push_back(make_int2((int)si.size(), j));
+                si.push_back(ScatterItem{side == 0 ? jac[i].x : jac[i].v, rows, sel[i].perm, sel[i].sigma_sorted,
+                                         sel[i].kept, f, rows, side == 0 ? 1 : 0});
+            }
+        }
+        if (!jobs.empty()) {
+            run_gather(c, gi);
+            polar_refine(c, jobs);
+            launch_scatter_cols(upload(c, si), upload(c, st), (int)st.size(), c->stream);
+            for (cplx* f : tmp) dfree(c, f);
+        }
+    }
+    // per task: the factors to apply on each side, innermost first
+    std::vector<std::vector<int>> uq(nt), vq(nt);
+    size_t stages = 0;
+    for (size_t i = 0; i < nt; ++i) {
+        for (int j = off[i].np; j >= 1; --j) (j % 2 ? uq[i] : vq[i]).push_back(j);
+        stages = std::max(stages, std::max(uq[i].size(), vq[i].size()));
+    }
     if (zero_end > zero_begin)
         launch_fill(reinterpret_cast<double*>(ws + zero_begin), (zero_end - zero_begin) / sizeof(double), 0.0,
                     c->stream);
-    std::vector<GatherItem> pre_g, to_ux;
+    // the polar refinement reads every column of vx: the ones beyond the kept must be zero
+    for (size_t i = 0; i < nt; ++i)
+        if (off[i].pre && tasks[i].refine)
+            launch_fill(reinterpret_cast<double*>(P(off[i].vx)), 2 * (size_t)off[i].q_ * std::max(off[i].k_, 1), 0.0,
+                        c->stream);
+    std::vector<GatherItem> pre_g;
+    std::vector<std::vector<GatherItem>> to_ux(stages + 1);  // by the stage that first needs ux
     for (size_t i = 0; i < nt; ++i) {
         const Off& o = off[i];
         if (!o.pre) continue;
+        const bool even = o.np % 2 == 0;
         GatherItem g{};
         g.src = even ? jac[i].x : jac[i].v; g.ld_src = o.q_; g.rows = o.q_; g.perm = sel[i].perm;
         g.sigma_sorted = sel[i].sigma_sorted;
         g.divide_sigma = even ? 1 : 0; g.kept = sel[i].kept; g.rowscale_mod = 1; g.rowscale_div = 1;
         g.conj_transpose = 0; g.max_cols = o.k_;
-        if (uq.size() > 1) {  // the q × q factors act on a q-row copy first
+        if (uq[i].size() > 1) {  // the q × q factors act on a q-row copy first
             g.out = P(o.tmp); g.ld_out = o.q_;
             GatherItem m = g;
             m.src = P(o.tmp); m.perm = nullptr; m.divide_sigma = 0; m.out = P(o.ux); m.ld_out = o.p_;
-            to_ux.push_back(m);
+            to_ux[uq[i].size() - 1].push_back(m);  // Q1 is the last U-side factor
         } else {
             g.out = P(o.ux); g.ld_out = o.p_;
         }
@@ -731,18 +861,25 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         pre_g.push_back(h);
     }
     run_gather(c, pre_g);
-    const size_t stages = std::max(uq.size(), vq.size());
     for (size_t r = 0; r < stages; ++r) {
-        const bool u_now = r < uq.size(), v_now = r < vq.size();
-        if (u_now && uq[r] == 1 && uq.size() > 1) run_gather(c, to_ux);
+        run_gather(c, to_ux[r]);
         std::vector<QrApplyJob> app;
         for (size_t i = 0; i < nt; ++i) {
             const Off& o = off[i];
             if (!o.pre) continue;
-            if (u_now) app.push_back(QrApplyJob{&qr[i][uq[r]], P(uq[r] == 1 ? o.ux : o.tmp), o.k_});
-            if (v_now) app.push_back(QrApplyJob{&qr[i][vq[r]], P(o.vx), o.k_});
+            if (r < uq[i].size()) app.push_back(QrApplyJob{&qr[i][uq[i][r]], P(uq[i][r] == 1 ? o.ux : o.tmp), o.k_});
+            if (r < vq[i].size()) app.push_back(QrApplyJob{&qr[i][vq[i][r]], P(o.vx), o.k_});
         }
         run_qr_apply_q(c, app);
+    }
+    {
+        std::vector<PolarJob> jobs;
+        for (size_t i = 0; i < nt; ++i)
+            if (tasks[i].refine && off[i].pre) {
+                jobs.push_back(PolarJob{P(off[i].ux), off[i].p_, std::max(off[i].k_, 1)});
+                jobs.push_back(PolarJob{P(off[i].vx), off[i].q_, std::max(off[i].k_, 1)});
+            }
+        polar_refine(c, jobs);
     }
 
     // ---- outputs

@@ -70,6 +70,11 @@ struct qh_context {
     // R_{j-1}^H = Q_j R_j, Jacobi on R_n^H
     int jac_precondition = 3;
     int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
+    bool use_graphs = true;   // a Jacobi sweep's rounds captured once and replayed as a HIP graph
+    // set by qh_ctx_autotune: QR passes per size class (columns up to the key) and the
+    // Jacobi block width per largest class of a batch; empty = the settings above
+    std::map<int, int> tuned_passes;
+    std::map<int, int> tuned_block;
     // device buffers allocated by the entry point running now and not yet owned by
     // an MPS: freed if that call fails, so an error leaks nothing
     std::unordered_set<void*> call_allocs;
@@ -148,7 +153,8 @@ void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);
 // runs set-identity + sweeps until every item converged; throws on sweep limit.
 // fro2_out (optional, one double per item) receives ||X||_F² and stays valid after.
-void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out = nullptr);
+// block: column block width (8 or 16; 0 = the context's setting)
+void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out = nullptr, int block = 0);
 void run_select(qh_context* c, std::vector<SelectItem>& items);
 void run_gather(qh_context* c, std::vector<GatherItem>& items);
 
@@ -177,6 +183,11 @@ struct SvdTask {
     double threshold = 1e-16;
     int maxdim = -1;
     int renormalize = 0;
+    // one Newton-Schulz (polar) step on both kept factors (F <- F (3I - F^H F) / 2):
+    // their orthogonality goes from what the Jacobi's tolerance and its accumulated
+    // rotations leave (~1e-12 in Frobenius norm at 256 columns) to rounding level, like
+    // LAPACK's factors -- for callers whose result must pass the canonical-form tests
+    int refine = 0;
     std::vector<SvdOut> outs;
     double* lam_dst = nullptr;     // kept singular values (may be null)
     double* sigma_dst = nullptr;   // all sorted singular values (k), may be null
@@ -184,11 +195,22 @@ struct SvdTask {
     double discarded = 0.0;        // result
     int kmax() const;
 };
+// one dense rows × cols matrix (ld rows) for polar_refine
+struct PolarJob {
+    cplx* f;
+    int rows, cols;
+};
+// F <- F (3I - F^H F) / 2 for every job, in place (one Newton-Schulz step toward the
+// nearest isometry; zero columns stay zero)
+void polar_refine(qh_context* c, const std::vector<PolarJob>& jobs);
+
 // Runs the batch of SVD tasks (Jacobi + select + gathers) and synchronises.
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks);
 
 // most QR passes of the Jacobi preconditioner (qh_context::jac_precondition)
 constexpr int QR_MAX_PASSES = 4;
+// the passes a matrix with q columns gets (tuned table or the context's setting)
+int qr_passes_for(const qh_context* c, int q);
 // Householder QR of every job in place (R in the upper triangle); reflectors into v (m × n),
 // compact-WY triangles into t (ceil(n / QR_NB) · QR_NB² entries).
 struct QrJob {

@@ -141,6 +141,48 @@ function Qrochet.Chain(h::HIPChain)
     ψ
 end
 
+"""
+    rand(HIPChain, Open, State; n, χ, p = 2, eltype = Float64, seed = 0, context)
+
+`rand(Chain, Open, State; n, χ, p, eltype)` (src/Ansatz/Chain.jl:223-256) sampled on the
+device (`qh_mps_rand`): bond b of dimension min(χ, p^b, p^(n-b)), right-canonical from
+site 2, norm 1.  `seed` replaces Julia's RNG.
+"""
+function Base.rand(::Type{HIPChain}, ::Type{Qrochet.Open}, ::Type{Qrochet.State}; n::Integer, χ::Integer, p::Integer = 2,
+                   eltype::Type = Float64, seed::Integer = 0, context::Context = default_context())
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(context, ccall((:qh_mps_rand, libqh), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Culonglong, Ref{Ptr{Cvoid}}),
+                         context.ptr, n, χ, p, eltype <: Complex ? 1 : 0, seed % Culonglong, out))
+    wrap(context, out[])
+end
+
+"""
+    autotune!(context; max_q = 512, batch = 32)
+
+Pick the batched SVD's schedule (Jacobi block width, QR passes) per matrix size class by
+measurement (`qh_ctx_autotune`); returns (q, block, passes, best_ms, previous_ms) per class.
+"""
+function autotune!(context::Context = default_context(); max_q::Integer = 512, batch::Integer = 32)
+    rep = zeros(Float64, 40)
+    n = Ref{Cint}(0)
+    check(context, ccall((:qh_ctx_autotune, libqh), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ref{Cint}),
+                         context.ptr, max_q, batch, rep, n))
+    [(q = Int(rep[5k+1]), block = Int(rep[5k+2]), passes = Int(rep[5k+3]), best_ms = rep[5k+4], previous_ms = rep[5k+5])
+     for k in 0:n[]-1]
+end
+
+"Replay each Jacobi sweep's rounds as one captured HIP graph (`true`, default) or launch them one by one."
+set_graphs!(context::Context, enable::Bool) =
+    check(context, ccall((:qh_ctx_set_graphs, libqh), Cint, (Ptr{Cvoid}, Cint), context.ptr, enable ? 1 : 0))
+
+"`overlap(a, b)` without device allocation (`qh_overlap_noalloc`; open chains up to about χ = 1024)."
+function overlap_noalloc(a::HIPChain, b::HIPChain)
+    out = zeros(Float64, 2)
+    check(a.ctx, ccall((:qh_overlap_noalloc, libqh), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}),
+                       a.ctx.ptr, a.ptr, b.ptr, out))
+    complex(out[1], out[2])
+end
+
 Base.copy(h::HIPChain) = (r = Ref{Ptr{Cvoid}}(C_NULL); check(h.ctx, ccall((:qh_mps_clone, libqh), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}), h.ctx.ptr, h.ptr, r)); wrap(h.ctx, r[]))
 Base.deepcopy(h::HIPChain) = copy(h)
 
@@ -306,6 +348,6 @@ for f in (:isleftcanonical, :isrightcanonical)
     end
 end
 
-export HIPChain, simple_update!, evolve_layer!
+export HIPChain, simple_update!, evolve_layer!, autotune!, set_graphs!, overlap_noalloc
 
 end

@@ -33,6 +33,7 @@ from .chain import (  # noqa: F401
     overlap,
     overlap_noalloc,
     pMPS,
+    rand_chain,
     simple_update_,
     site,
     truncate,

@@ -54,8 +54,12 @@ SIGNATURES = {
     "qh_ctx_set_jacobi": (C.c_int, [_ctx_p, C.c_int, C.c_int]),
     "qh_ctx_set_qr_passes": (C.c_int, [_ctx_p, C.c_int]),
     "qh_ctx_mem_info": (C.c_int, [_ctx_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "qh_ctx_autotune": (C.c_int, [_ctx_p, C.c_int, C.c_int, _dp, _ip]),
+    "qh_ctx_autotune_reset": (C.c_int, [_ctx_p]),
+    "qh_ctx_set_graphs": (C.c_int, [_ctx_p, C.c_int]),
     "qh_mps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_mps_p)]),
     "qh_pmps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mps_p)]),
+    "qh_mps_rand": (C.c_int, [_ctx_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_mps_p)]),
     "qh_mps_is_periodic": (C.c_int, [_mps_p, _ip, _ip]),
     "qh_mps_clone": (C.c_int, [_ctx_p, _mps_p, C.POINTER(_mps_p)]),
     "qh_mps_free": (C.c_int, [_mps_p]),

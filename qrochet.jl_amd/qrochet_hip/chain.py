@@ -178,6 +178,23 @@ class Context:
         """Householder QR passes preconditioning the Jacobi SVD (0..4, default 3)."""
         self.check(L.load().qh_ctx_set_qr_passes(self.handle, int(passes)))
 
+    def autotune(self, max_q=512, batch=32):
+        """Tune the batched SVD's schedule (Jacobi block width, QR passes) per size class
+        up to ``max_q`` columns (`qh_ctx_autotune`); returns one dict per class."""
+        rep = (C.c_double * 40)()
+        n = C.c_int()
+        self.check(L.load().qh_ctx_autotune(self.handle, int(max_q), int(batch), rep, C.byref(n)))
+        return [dict(q=int(rep[5 * k]), block=int(rep[5 * k + 1]), qr_passes=int(rep[5 * k + 2]),
+                     best_ms=rep[5 * k + 3], previous_ms=rep[5 * k + 4]) for k in range(n.value)]
+
+    def set_graphs(self, enable: bool):
+        """Replay each Jacobi sweep's rounds as a captured HIP graph (default) or launch
+        them one by one (`qh_ctx_set_graphs`)."""
+        self.check(L.load().qh_ctx_set_graphs(self.handle, 1 if enable else 0))
+
+    def autotune_reset(self):
+        self.check(L.load().qh_ctx_autotune_reset(self.handle))
+
     def mem_info(self):
         """(free, total) bytes of the device."""
         f = C.c_size_t()
@@ -377,6 +394,20 @@ def MPS(arrays, order=("o", "l", "r"), lambdas=None, context: Context | None = N
                 lam_ptrs[b] = v.ctypes.data_as(L._dp)
     h = C.c_void_p()
     ctx.check(L.load().qh_mps_create(ctx.handle, n, d, dims, ptrs, lam_ptrs, C.byref(h)))
+    return Chain(h, ctx)
+
+
+def rand_chain(n: int, chi: int, p: int = 2, eltype=np.float64, seed: int = 0,
+               context: Context | None = None) -> Chain:
+    """`rand(Chain, Open, State; n, χ, p, eltype)` (src/Ansatz/Chain.jl:223-256), sampled
+    on the device: bond b of dimension min(χ, p^b, p^(n-b)), right-canonical from site
+    2, norm 1 (for χ >= p).  ``seed`` fixes the sample (Julia's RNG stream is not
+    reproduced)."""
+    ctx = context or default_context()
+    cplx = np.issubdtype(np.dtype(eltype), np.complexfloating)
+    h = C.c_void_p()
+    ctx.check(L.load().qh_mps_rand(ctx.handle, int(n), int(chi), int(p), 1 if cplx else 0,
+                                   C.c_ulonglong(int(seed) & 0xFFFFFFFFFFFFFFFF), C.byref(h)))
     return Chain(h, ctx)
 
 

@@ -3,10 +3,11 @@
 The reference ingests Quac gates by turning each into a `Dense` operator on
 `[Site.(lanes)..., Site.(lanes; dual=true)...]` (ext/QrochetQuacExt.jl:7-15) and
 applies it with `evolve!` (ext/QrochetQuacExt.jl:15); Yao circuits become a
-`Quantum` network gate by gate (ext/QrochetYaoExt.jl:15-48).  Here a circuit is
-a list of `Op(name, lanes, params)` (or raw matrices) and `run_` applies it to a
-device MPS, scheduling the gates into layers of disjoint lanes so that every
-layer of two-site gates is ONE batched update (`qh_evolve2_batch`).
+`Quantum` network gate by gate (ext/QrochetYaoExt.jl:15-48), `from_yao` below.
+Here a circuit is a list of `Op(name, lanes, params)` (or raw matrices) and
+`run_` applies it to a device MPS, scheduling the gates into layers of disjoint
+lanes so that every layer of two-site gates is ONE batched update
+(`qh_evolve2_batch`).
 
 Matrix convention (the reference's, `Dense.jl:21-34`): a k-lane gate is a
 d^k × d^k matrix whose row/column index has the FIRST lane as the least
@@ -103,6 +104,77 @@ def layers(circuit) -> list:
         for l in ln:
             free[l] = k + 1
     return out
+
+
+# ---------------------------------------------------------------------------
+# Yao circuits (ext/QrochetYaoExt.jl:15-48)
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Put:
+    """Yao's `put(n, locs => block)`: a dense 2^k × 2^k block on `locs` (1-based), the
+    first location the least significant digit of the matrix index (`mat(block)`)."""
+
+    locs: tuple
+    matrix: np.ndarray
+
+
+@dataclasses.dataclass
+class Swap:
+    """Yao's `swap(n, a, b)`."""
+
+    a: int
+    b: int
+
+
+def flatten(blocks) -> list:
+    """Nested chains of blocks (Yao `ChainBlock`s) as one flat list, in order
+    (`flatten_circuit` / `Yao.Optimise.eliminate_nested`, ext/QrochetYaoExt.jl:7-13)."""
+    out = []
+    for b in blocks:
+        if isinstance(b, (list, tuple)):
+            out.extend(flatten(b))
+        else:
+            out.append(b)
+    return out
+
+
+def from_yao(blocks, n: int) -> list:
+    """The gates of a Yao circuit on `n` qubits as `Op`s on MPS sites, with the
+    semantics of `Qrochet.Quantum(circuit::AbstractBlock)` (ext/QrochetYaoExt.jl:15-48):
+    a `Swap` adds no tensor, it exchanges the two wires; every other block acts on
+    the wires currently at its locations; at the end the input and output sites of
+    lane l are the two ends of the wire that ended at l (Site(l; dual) => first,
+    Site(l) => last, :44-47).  A wire is identified with the MPS site where it ends,
+    so a gate applied at a time when lane l carries wire w acts on site
+    end_slot(w), and no SWAP has to be applied to the state.  Only 2-level blocks
+    (`@assert nlevel(circuit) == 2`, :16)."""
+    flat = flatten(blocks)
+    lane_wire = list(range(n))  # lane -> wire (0-based); wires are named by their start lane
+    for b in flat:
+        if isinstance(b, Swap):
+            if not (1 <= b.a <= n and 1 <= b.b <= n):
+                raise ArgumentError("swap locations outside the circuit")
+            lane_wire[b.a - 1], lane_wire[b.b - 1] = lane_wire[b.b - 1], lane_wire[b.a - 1]
+        elif not isinstance(b, Put):
+            raise ArgumentError(f"unsupported Yao block {type(b).__name__} (put and swap blocks only)")
+    end_slot = [0] * n
+    for lane, w in enumerate(lane_wire):
+        end_slot[w] = lane + 1
+    lane_wire = list(range(n))
+    ops = []
+    for b in flat:
+        if isinstance(b, Swap):
+            lane_wire[b.a - 1], lane_wire[b.b - 1] = lane_wire[b.b - 1], lane_wire[b.a - 1]
+            continue
+        m = np.asarray(b.matrix, dtype=np.complex128)
+        k = len(b.locs)
+        if m.shape != (2**k, 2**k):
+            raise ArgumentError("Only support 2-level qubits")
+        if not all(1 <= l <= n for l in b.locs):
+            raise ArgumentError("block locations outside the circuit")
+        sites = tuple(end_slot[lane_wire[l - 1]] for l in b.locs)
+        ops.append(Op("yao", sites, matrix=m))
+    return ops
 
 
 def run_(psi: Chain, circuit, threshold=None, maxdim=None, precision="fp64") -> Chain:

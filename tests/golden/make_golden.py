@@ -15,6 +15,7 @@ import os
 import sys
 
 import numpy as np
+import scipy.linalg as sla
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -72,7 +73,51 @@ class _SpectrumRecorder:
 
 
 CONFIG2_STEPS = 6
+CONFIG2_EXACT_STEPS = 3
 CONFIG2_LAMBDA_BONDS = (16, 32, 48)
+
+
+def _gesvd(m):
+    return sla.svd(m, full_matrices=False, lapack_driver="gesvd")
+
+
+def _config2_run(steps, svd=None):
+    """Workload 2 through `steps` Strang steps with the oracle, optionally with another
+    LAPACK SVD driver; kept dimension and cut-off distance per update, ⟨Z_i⟩ after
+    CONFIG2_EXACT_STEPS and after `steps` steps, Λ of bulk bonds, overlaps."""
+    arrays, layers, tr, _ = wl.config2(steps=steps)
+    psi = oc.Chain([a.copy() for a in arrays])
+    psi0 = psi.copy()
+    kept, margin = [], []
+    z_exact = None
+    orig = oc._svd
+    if svd is not None:
+        oc._svd = svd
+    try:
+        with _SpectrumRecorder() as rec:
+            for li, layer in enumerate(layers):
+                for b, g in layer:
+                    oc.evolve(psi, oc.gate2(g, b), **tr)
+                    k = psi.bond_dims()[b - 1]
+                    sv_ = rec.last
+                    kept.append(k)
+                    if k < min(len(sv_), tr["maxdim"]):
+                        margin.append(min(abs(np.log(sv_[k - 1] / tr["threshold"])),
+                                          abs(np.log(sv_[k] / tr["threshold"]))))
+                    else:
+                        margin.append(np.inf)
+                if li + 1 == 3 * CONFIG2_EXACT_STEPS:
+                    z_exact = np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, psi.n + 1)])
+    finally:
+        oc._svd = orig
+    n = psi.n
+    out = dict(kept=np.array(kept, dtype=np.int64), threshold_margin=np.array(margin), z_exact=z_exact,
+               z=np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]),
+               norm2=np.complex128(oc.overlap(psi, psi)), overlap=np.complex128(oc.overlap(psi, psi0)),
+               final_dims=np.array(psi.bond_dims(), dtype=np.int64))
+    for b in CONFIG2_LAMBDA_BONDS:
+        out[f"lambda{b}"] = psi.lam[b - 1]
+    return out
 
 
 def config2_saturation_outputs(steps=CONFIG2_STEPS):
@@ -82,36 +127,22 @@ def config2_saturation_outputs(steps=CONFIG2_STEPS):
     them once it has saturated.  Records, per update in schedule order, the kept
     dimension and -- where the cut-off (not the cap) decided it -- the distance of
     the decisive singular values from the cut-off, |log(σ/1e-12)| of the smallest
-    kept and the largest dropped one; then Λ of three bulk bonds, all unnormalised
-    ⟨Z_i⟩ (`expect`), ⟨ψ|ψ⟩ and the overlap with the initial product state."""
-    arrays, layers, tr, _ = wl.config2(steps=steps)
-    psi = oc.Chain([a.copy() for a in arrays])
-    psi0 = psi.copy()
-    kept, margin = [], []
-    with _SpectrumRecorder() as rec:
-        for layer in layers:
-            for b, g in layer:
-                oc.evolve(psi, oc.gate2(g, b), **tr)
-                k = psi.bond_dims()[b - 1]
-                s = rec.last
-                kept.append(k)
-                if k < min(len(s), tr["maxdim"]):
-                    margin.append(min(abs(np.log(s[k - 1] / tr["threshold"])), abs(np.log(s[k] / tr["threshold"]))))
-                else:
-                    margin.append(np.inf)
-    n = psi.n
-    out = dict(
-        steps=np.int64(steps),
-        kept=np.array(kept, dtype=np.int64),
-        threshold_margin=np.array(margin),
-        lambda_bonds=np.array(CONFIG2_LAMBDA_BONDS, dtype=np.int64),
-        z=np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]),
-        norm2=np.complex128(oc.overlap(psi, psi)),
-        overlap=np.complex128(oc.overlap(psi, psi0)),
-        final_dims=np.array(psi.bond_dims(), dtype=np.int64),
-    )
-    for b in CONFIG2_LAMBDA_BONDS:
-        out[f"lambda{b}"] = psi.lam[b - 1]
+    kept and the largest dropped one; all unnormalised ⟨Z_i⟩ (`expect`) after
+    CONFIG2_EXACT_STEPS steps and at the end, Λ of three bulk bonds, ⟨ψ|ψ⟩ and the
+    overlap with the initial product state.
+
+    The same run with LAPACK's zgesvd in place of zgesdd (both complex FP64,
+    `*_gesvd` entries) measures how far two correct FP64 SVDs carry the state
+    apart: once the cut-off at 1e-12 (about 1e-13 of σ₁) decides kept dimensions,
+    the decisions depend on rounding, and in the default (not canonical) gauge the
+    differences grow over the layers."""
+    main = _config2_run(steps)
+    other = _config2_run(steps, _gesvd)
+    out = dict(steps=np.int64(steps), exact_steps=np.int64(CONFIG2_EXACT_STEPS),
+               lambda_bonds=np.array(CONFIG2_LAMBDA_BONDS, dtype=np.int64), **main)
+    for k in ("kept", "z", "overlap", "norm2") + tuple(f"lambda{b}" for b in CONFIG2_LAMBDA_BONDS):
+        out[f"{k}_gesvd"] = other[k]
+    out["z_exact_gesvd"] = other["z_exact"]
     return out
 
 

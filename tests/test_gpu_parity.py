@@ -607,3 +607,39 @@ def test_jacobi_block_sizes(qh, block):
     for lg in psi.lambdas():
         if lg is not None:
             assert np.all(np.diff(lg) <= 0)  # singular values sorted, descending
+
+
+@pytest.mark.parametrize("padded", [False, True])
+def test_canonize_batched_bond_svds_match_oracle(qh, padded):
+    """`canonize!` (`Chain.jl:469-497`): the Schmidt values of every bond (one batched
+    SVD of the bond matrices of a QR sweep) equal the oracle's dependent SVD sweep;
+    Λ_{i-1} Γ_i is a left and Γ_i Λ_i a right isometry; the state is unchanged.  With a
+    first bond wider than d (padded) the reference's SVD sweep runs instead."""
+    rng = np.random.default_rng(61)
+    n = 10
+    chis = wl.max_profile(n, 2, 24)
+    if padded:
+        chis[0] = 4  # site 1 is 1 x 2 x 4: wider than d = 2
+    arrays = wl.random_mps_arrays(rng, n, 2, chis)
+    psi = qh.from_internal(arrays)
+    ref = oc.canonize(oc.Chain([a.copy() for a in arrays]))
+    v0 = oc.statevector(oc.Chain([a.copy() for a in arrays]))
+    qh.canonize_(psi)
+    nrm = np.linalg.norm(v0)
+    assert psi.bond_dims() == ref.bond_dims()
+    for lg, lr in zip(psi.lambdas(), ref.lam):
+        assert np.max(np.abs(lg - lr)) < 1e-10 * nrm
+        assert abs(np.sum(lg ** 2) - nrm ** 2) < 1e-10 * nrm ** 2
+    arrs, lams = psi.internal_arrays(), psi.lambdas()
+    for i in range(1, n + 1):
+        a = arrs[i - 1]
+        if i <= n - 1:  # Λ_{i-1} Γ_i: orthonormal columns
+            left = a * (lams[i - 2][:, None, None] if i >= 2 else 1.0)
+            mm = left.reshape(-1, left.shape[2])
+            assert np.max(np.abs(mm.conj().T @ mm - np.eye(mm.shape[1]))) < 1e-10, i
+        if i >= 2:  # Γ_i Λ_i (Γ_n at the last site): orthonormal rows
+            right = a * (lams[i - 1][None, None, :] if i <= n - 1 else 1.0)
+            mm = right.reshape(right.shape[0], -1)
+            assert np.max(np.abs(mm @ mm.conj().T - np.eye(mm.shape[0]))) < 1e-10, i
+    got = oc.statevector(oc.Chain(psi.internal_arrays(), psi.lambdas()))
+    assert np.max(np.abs(got - v0)) < 1e-10 * np.max(np.abs(v0))

@@ -2,10 +2,12 @@
 against the FP64 oracle.  The SVD, truncation and everything after θ stay FP64.
 
 Stated tolerances (bounds asserted here; the measured errors are printed):
-  * FP32 operands, FP32 accumulation: state vector of an untruncated evolution
-    within 1e-5 (relative to ‖ψ‖ = 1), ⟨Z_i⟩ of workload 1 (60 updates) within 1e-5;
-  * bf16x3 (hi/lo bfloat16 split, three products, FP32 accumulation): within
-    1e-3 for both — its operands carry 16 significant bits.
+  * FP32 operands and products on the matrix cores, FP32 accumulation within
+    16-wide chunks of the contracted index and the chunk sums in FP64: the
+    mixed-precision bound 1e-6, on the state vector of an untruncated evolution
+    (relative to ‖ψ‖ = 1) and on ⟨Z_i⟩ of workload 1 (60 updates);
+  * bf16x3 (hi/lo bfloat16 split, three products): 1e-4 for both -- its operands
+    carry 16 significant bits, so it cannot meet 1e-6.
 Kept bond dimensions are identical to the FP64 path on these full-rank cases.
 """
 import numpy as np
@@ -17,7 +19,7 @@ from oracle import workloads as wl
 
 pytestmark = pytest.mark.gpu
 
-BOUND = {"fp32": 1e-5, "bf16x3": 1e-3}
+BOUND = {"fp32": 1e-6, "bf16x3": 1e-4}
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
@@ -78,4 +80,6 @@ def test_reduced_precision_vidal_and_batched(qh, precision):
         assert np.array_equal(x, y)
     va = oc.statevector(oc.Chain(a.internal_arrays(), a.lambdas()))
     vr = oc.statevector(ref)
-    assert np.max(np.abs(va - vr)) < BOUND[precision]
+    err = np.max(np.abs(va - vr))
+    print(f"{precision}: Vidal mode, batched, max |psi - psi_fp64| = {err:.3e}")
+    assert err < BOUND[precision]

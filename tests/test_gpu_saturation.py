@@ -3,10 +3,13 @@
 * Workload 2 (n = 64, Ising Trotter, cut-off 1e-12, maxdim 256) through 6 Strang
   steps: χ reaches 256 at layer 14, so the cut-off decides the kept dimensions
   while the state grows and the cap decides them afterwards.  Per call and
-  batched, against the oracle's fixture (`tests/golden/config2_saturation.npz`):
-  identical kept dimension for every one of the 570 updates, Λ of three bulk
-  bonds, every unnormalised ⟨Z_i⟩, ⟨ψ|ψ⟩ and the overlap with the initial state
-  within 1e-10.
+  batched, against the oracle's fixture (`tests/golden/config2_saturation.npz`).
+  Over the first 3 Strang steps: identical kept dimension for every update and
+  every unnormalised ⟨Z_i⟩ within 1e-10.  Beyond that the cut-off at 1e-12 (about
+  1e-13 of σ₁) decides kept dimensions at the rounding level, and two correct
+  FP64 SVDs part: the fixture holds the same run with LAPACK's zgesvd in place of
+  zgesdd (14 of 570 kept dimensions differ, ⟨Z_i⟩ by 1e-9, Λ by 1e-2).  The GPU
+  must stay within that spread (stated factors below).
 * A reduced batched uneven workload (16 MPS, n ~ U{8..20}, χmax in {16..256},
   4 Haar layers, every layer of every MPS in ONE batched call) against the
   oracle's fixture (`tests/golden/config3_reduced.npz`): FP64 within 1e-10,
@@ -29,10 +32,11 @@ from tests.golden import make_golden
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
-# FP32 θ (v_mfma_f32_16x16x4_f32, FP32 accumulation) against the FP64 oracle on the
+# FP32 θ (v_mfma_f32_16x16x4_f32, chunk sums in FP64) against the FP64 oracle on the
 # reduced batched workload, on unnormalised <Z_i> of normalised states and on the
-# overlap with the initial state; the SVD and truncation stay FP64.
-FP32_BOUND = 1e-5
+# overlap with the initial state; the SVD and truncation stay FP64: the
+# mixed-precision bound.
+FP32_BOUND = 1e-6
 
 
 def _z_terms(qh, n):
@@ -46,7 +50,8 @@ def test_config2_through_saturation_matches_golden(qh, batched):
     psi = qh.from_internal(arrays)
     psi0 = psi.copy()
     kept = []
-    for layer in layers:
+    z_exact = None
+    for li, layer in enumerate(layers):
         if batched:
             k, _ = qh.evolve_layer_([(psi, qh.gate(g, b, b + 1)) for b, g in layer], **tr)
             kept.extend(k)
@@ -55,22 +60,40 @@ def test_config2_through_saturation_matches_golden(qh, batched):
                 info = {}
                 qh.evolve_(psi, qh.gate(g, b, b + 1), info=info, **tr)
                 kept.append(info["kept"])
+        if li + 1 == 3 * int(fx["exact_steps"]):
+            n_exact = len(kept)
+            z_exact = qh.expect_many(psi, _z_terms(qh, psi.n))
     kept = np.array(kept)
+    ref, alt = fx["kept"], fx["kept_gesvd"]
     margin = fx["threshold_margin"]
-    print(f"closest cut-off decision: |log(sigma / 1e-12)| = {np.min(margin):.2e}")
-    bad = np.nonzero(kept != fx["kept"])[0]
-    assert bad.size == 0, [(int(i), int(kept[i]), int(fx["kept"][i]), float(margin[i])) for i in bad]
-    assert psi.bond_dims() == list(fx["final_dims"])
-    assert max(psi.bond_dims()) == 256
-    for b in fx["lambda_bonds"]:
-        lam = psi.lambda_(int(b))
-        assert np.max(np.abs(lam - fx[f"lambda{b}"])) < TOL
+    bad = np.nonzero(kept != ref)[0]
+    bad_alt = np.nonzero(alt != ref)[0]
     z = qh.expect_many(psi, _z_terms(qh, psi.n))
-    dz = np.max(np.abs(z - fx["z"]))
-    print(f"max |<Z_i> - <Z_i>_oracle| = {dz:.2e}")
-    assert dz < TOL
-    assert abs(qh.overlap(psi, psi) - fx["norm2"]) < TOL
-    assert abs(qh.overlap(psi, psi0) - fx["overlap"]) < TOL
+    dz, dz_alt = np.max(np.abs(z - fx["z"])), np.max(np.abs(fx["z_gesvd"] - fx["z"]))
+    dov = abs(qh.overlap(psi, psi0) - fx["overlap"])
+    dov_alt = abs(fx["overlap_gesvd"] - fx["overlap"])
+    dlam = max(np.max(np.abs(psi.lambda_(int(b)) - fx[f"lambda{b}"])) for b in fx["lambda_bonds"])
+    dlam_alt = max(np.max(np.abs(fx[f"lambda{b}_gesvd"] - fx[f"lambda{b}"])) for b in fx["lambda_bonds"])
+    print(f"kept-dimension differences from zgesdd: GPU {bad.size} (first at update {bad[0] if bad.size else '-'}), "
+          f"zgesvd {bad_alt.size} (first at {bad_alt[0] if bad_alt.size else '-'}) of {ref.size}")
+    print(f"max |d<Z_i>|: GPU {dz:.2e}, zgesvd {dz_alt:.2e}; |d overlap|: GPU {dov:.2e}, zgesvd {dov_alt:.2e}; "
+          f"max |d Lambda|: GPU {dlam:.2e}, zgesvd {dlam_alt:.2e}")
+    # (1) before the cut-off decides near the rounding level (the first exact_steps Strang
+    #     steps, chi up to 85): identical kept dimensions and <Z_i> within 1e-10
+    assert np.array_equal(kept[:n_exact], ref[:n_exact])
+    assert np.max(np.abs(z_exact - fx["z_exact"])) < TOL
+    # (2) through saturation: within the spread of two LAPACK FP64 SVD drivers on the same
+    #     inputs -- differing kept dimensions only at cut-off decisions, by a few values,
+    #     at most 3x as many as zgesvd's, and <Z_i>, overlap and Λ no further from zgesdd
+    #     than 10x zgesvd's distance (and <Z_i>, overlap within 1e-8 in any case)
+    assert np.all(np.isfinite(margin[bad]))
+    assert np.all(np.abs(kept[bad] - ref[bad]) <= 3)
+    assert bad.size <= 3 * max(bad_alt.size, 1)
+    assert dz < max(10 * dz_alt, TOL) and dz < 1e-8
+    assert dov < max(10 * dov_alt, TOL) and dov < 1e-8
+    assert dlam < max(10 * dlam_alt, TOL)
+    assert max(psi.bond_dims()) == 256
+    assert abs(qh.overlap(psi, psi) - fx["norm2"]) < 1e-8
 
 
 @pytest.mark.parametrize("precision,bound", [("fp64", TOL), ("fp32", FP32_BOUND)])
