@@ -94,6 +94,11 @@ int qh_ctx_set_qr_passes(qh_context* ctx, int passes);
  * default) the rounds of a sweep are captured once per batched SVD and replayed as
  * one graph launch per sweep; 0 launches them one by one.  The results are the same. */
 int qh_ctx_set_graphs(qh_context* ctx, int enable);
+/* Column pivoting of the SVD's preconditioner (no reference counterpart): with
+ * enable = 1 (the default) the first QR pass factors the matrix with its columns
+ * ordered by decreasing norm, X P = Q1 R1, which grades R1 and saves Jacobi sweeps;
+ * 0 factors X as it is.  Changes the rounding of the results only. */
+int qh_ctx_set_pivoting(qh_context* ctx, int enable);
 /* Schedule tuner of the batched SVD (no reference counterpart; the project's
  * autotuner): for each size class q = 64, 128, ... <= max_q (at most 4096) it times
  * the batched SVD of `batch` synthetic q × q matrices (flat and decaying spectra;

@@ -770,75 +770,143 @@ __device__ __forceinline__ double splitmix_unit(unsigned long long k) {
 }
 
 constexpr int COMPLETE_MAXQ = SELECT_MAXQ;
-__global__ __launch_bounds__(256) void k_complete(const CompleteItem* __restrict__ items) {
+// The kept columns at or below the deflation floor (sorted positions [first, kept);
+// σ sorted decreasing, so first = the number of kept σ² above the floor) are replaced
+// by an orthonormal completion: deterministic pseudo-random starts, two passes of
+// classical Gram-Schmidt against the columns above the floor (one workgroup per
+// frozen column, all in parallel); the host then orthonormalises them among
+// themselves by a Householder QR.  Sorted column i is x[:, perm[i]] / s_i with
+// s_i = σ_i, or 1 where σ_i = 0.
+__device__ __forceinline__ int complete_first(const CompleteItem& it, int kept, double floor2) {
+    int lo = 0, hi = kept;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const double s = it.sigma_sorted[mid];
+        if (s * s > floor2) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ const double2* complete_col(const CompleteItem& it, int i) {
+    return it.x + (size_t)it.p * it.perm[i];
+}
+__device__ __forceinline__ double complete_inv(const CompleteItem& it, int i) {
+    const double s = it.sigma_sorted[i];
+    return s > 0.0 ? 1.0 / s : 1.0;
+}
+// sDot[i - i0] = <x_i, z> / s_i² for i in [i0, i1): a wave per column, four columns
+// per wave in flight, lanes striding the rows
+__device__ __forceinline__ void complete_dots(const CompleteItem& it, const double2* z, int i0, int i1,
+                                              double2* sDot) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = i0 + wv; i < i1; i += 16) {
+        double2 acc[4] = {cz(), cz(), cz(), cz()};
+        const double2* cc[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ok[u] = i + 4 * u < i1;
+            cc[u] = complete_col(it, ok[u] ? i + 4 * u : i);
+        }
+        for (int r = lane; r < it.p; r += 64) {
+            const double2 zr = z[r];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ok[u]) acc[u] = cfmac(cc[u][r], zr, acc[u]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[u].x += __shfl_xor(acc[u].x, off, 64);
+                acc[u].y += __shfl_xor(acc[u].y, off, 64);
+            }
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ok[u]) {
+                    const double w = complete_inv(it, i + 4 * u);
+                    sDot[i + 4 * u - i0] = cscale(acc[u], w * w);
+                }
+    }
+}
+// z <- z - Σ_{i in [i0, i1)} sDot[i - i0] x_i, a thread per row
+__device__ __forceinline__ void complete_update(const CompleteItem& it, double2* z, int i0, int i1,
+                                                const double2* sDot) {
+    for (int r = threadIdx.x; r < it.p; r += blockDim.x) {
+        double2 v = z[r];
+        int i = i0;
+        for (; i + 4 <= i1; i += 4) {
+            const double2 x0 = complete_col(it, i)[r], x1 = complete_col(it, i + 1)[r];
+            const double2 x2 = complete_col(it, i + 2)[r], x3 = complete_col(it, i + 3)[r];
+            v = csub(v, cmul(sDot[i - i0], x0));
+            v = csub(v, cmul(sDot[i + 1 - i0], x1));
+            v = csub(v, cmul(sDot[i + 2 - i0], x2));
+            v = csub(v, cmul(sDot[i + 3 - i0], x3));
+        }
+        for (; i < i1; ++i) v = csub(v, cmul(sDot[i - i0], complete_col(it, i)[r]));
+        z[r] = v;
+    }
+}
+
+// phase 1, workgroup (item, j): a random start for frozen column j, orthogonalised
+// twice against the columns above the floor
+__global__ __launch_bounds__(256) void k_complete_project(const CompleteItem* __restrict__ items,
+                                                          const int2* __restrict__ tiles) {
     __shared__ double2 sDot[COMPLETE_MAXQ];
-    __shared__ double sRed[4];
-    __shared__ int sFirst;
-    const CompleteItem it = items[blockIdx.x];
-    const int tid = threadIdx.x;
+    const int2 tt = tiles[blockIdx.x];
+    const CompleteItem it = items[tt.x];
+    const int j = tt.y;
     const int kept = *it.kept;
+    if (j >= kept) return;
     const double floor2 = it.floor_rel2 * (*it.fro2);
-    if (tid == 0) {
-        int f = kept;
-        while (f > 0 && it.sigma_sorted[f - 1] * it.sigma_sorted[f - 1] <= floor2) --f;
-        sFirst = f;
+    const double sj = it.sigma_sorted[j];
+    if (sj * sj > floor2) return;
+    const int first = complete_first(it, kept, floor2);
+    double2* z = const_cast<double2*>(complete_col(it, j));
+    // a deterministic pseudo-random start (the same for a matrix alone or in a batch)
+    for (int r = threadIdx.x; r < it.p; r += 256) {
+        const unsigned long long key = ((unsigned long long)j << 32) | (unsigned)r;
+        z[r] = make_double2(splitmix_unit(2 * key), splitmix_unit(2 * key + 1));
     }
     __syncthreads();
-    const int first = sFirst;
-    if (first >= kept) return;
-    const int p = it.p;
-    // sorted column i is x[:, perm[i]] / σ_i (or x[:, perm[i]] when σ_i = 0)
-    auto col = [&](int i) { return it.x + (size_t)p * it.perm[i]; };
-    auto inv = [&](int i) {
-        const double s = it.sigma_sorted[i];
-        return s > 0.0 ? 1.0 / s : 1.0;
-    };
-    for (int j = first; j < kept; ++j) {
-        double2* z = col(j);
-        // a deterministic pseudo-random start (the same for a matrix alone or in a batch)
-        for (int r = tid; r < p; r += 256) {
-            const unsigned long long key = ((unsigned long long)j << 32) | (unsigned)r;
-            z[r] = make_double2(splitmix_unit(2 * key), splitmix_unit(2 * key + 1));
-        }
+    for (int pass = 0; pass < 2 && first > 0; ++pass) {
+        complete_dots(it, z, 0, first, sDot);
         __syncthreads();
-        // two passes of classical Gram-Schmidt against every column before it
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int i = tid; i < j; i += 256) {
-                const double2* ci = col(i);
-                double2 acc = cz();
-                for (int r = 0; r < p; ++r) acc = cfmac(ci[r], z[r], acc);
-                sDot[i] = cscale(acc, inv(i));
-            }
-            __syncthreads();
-            for (int r = tid; r < p; r += 256) {
-                double2 v = z[r];
-                for (int i = 0; i < j; ++i) v = csub(v, cmul(sDot[i], cscale(col(i)[r], inv(i))));
-                z[r] = v;
-            }
-            __syncthreads();
-        }
-        double s = 0.0;
-        for (int r = tid; r < p; r += 256) s += cabs2(z[r]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if ((tid & 63) == 0) sRed[tid >> 6] = s;
-        __syncthreads();
-        const double nrm = sqrt((sRed[0] + sRed[1]) + (sRed[2] + sRed[3]));
-        const double sj = it.sigma_sorted[j];
-        const bool tiny = !(sj >= 1e-280);
-        const double f = (tiny ? 1.0 : sj) / nrm;
-        for (int r = tid; r < p; r += 256) z[r] = cscale(z[r], f);
-        __syncthreads();
-        if (tid == 0 && tiny) {
-            it.sigma_sorted[j] = 0.0;
-            it.lam_out[j] = 0.0;
-        }
+        complete_update(it, z, 0, first, sDot);
         __syncthreads();
     }
 }
 
-void launch_complete(const CompleteItem* d_items, int count, hipStream_t s) {
-    if (count > 0) k_complete<<<count, 256, 0, s>>>(d_items);
+// per item: counts[2i] = first (the first frozen sorted position), counts[2i + 1] = kept
+__global__ void k_complete_count(const CompleteItem* __restrict__ items, int count, int* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const CompleteItem it = items[i];
+    const int kept = *it.kept;
+    counts[2 * i] = complete_first(it, kept, it.floor_rel2 * (*it.fro2));
+    counts[2 * i + 1] = kept;
+}
+
+// frozen positions with σ below 1e-280 get σ = 0 and Λ = 0 (their completed columns are
+// stored as unit vectors; the others as unit vectors times σ)
+__global__ void k_complete_zero_tiny(const CompleteItem* __restrict__ items, const int* __restrict__ counts) {
+    const CompleteItem it = items[blockIdx.x];
+    for (int j = counts[2 * blockIdx.x] + threadIdx.x; j < counts[2 * blockIdx.x + 1]; j += blockDim.x)
+        if (!(it.sigma_sorted[j] >= 1e-280)) {
+            it.sigma_sorted[j] = 0.0;
+            it.lam_out[j] = 0.0;
+        }
+}
+
+void launch_complete_project(const CompleteItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles > 0) k_complete_project<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+void launch_complete_count(const CompleteItem* d_items, int count, int* d_counts, hipStream_t s) {
+    if (count > 0) k_complete_count<<<(count + 255) / 256, 256, 0, s>>>(d_items, count, d_counts);
+}
+void launch_complete_zero_tiny(const CompleteItem* d_items, const int* d_counts, int count, hipStream_t s) {
+    if (count > 0) k_complete_zero_tiny<<<count, 256, 0, s>>>(d_items, d_counts);
 }
 
 __global__ __launch_bounds__(256) void k_scatter_cols(const ScatterItem* __restrict__ items,
@@ -876,7 +944,7 @@ __global__ __launch_bounds__(256) void k_gather(const GatherItem* __restrict__ i
         double2 v = cz();
         if (j < kept && c < g.rows) {
             const int src_col = g.perm ? g.perm[j] : j;
-            v = g.src[c + (size_t)g.ld_src * src_col];
+            v = g.src[(g.row_src ? g.row_src[c] : c) + (size_t)g.ld_src * src_col];
             double f = 1.0;
             if (g.divide_sigma) {
                 const double sg = g.sigma_sorted[j];

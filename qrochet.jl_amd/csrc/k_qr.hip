@@ -653,4 +653,37 @@ void launch_upper_h_batch(const UpperHItem* d_items, const int2* d_tiles, int nt
     k_upper_h_batch<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
 }
 
+// Column order by decreasing norm (the column pivoting of the first pass, taken from
+// the initial norms): one 1024-thread workgroup per item, a wave per column for the
+// norms, then a rank sort with one thread per column.  A NaN norm sorts last (the
+// Jacobi reports the matrix as non-finite).
+__global__ __launch_bounds__(1024) void k_pivot_order(const PivotItem* __restrict__ items) {
+    __shared__ double sKey[PIVOT_MAXQ];
+    const PivotItem it = items[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int j = wv; j < it.q; j += 16) {
+        const cplx* col = it.x + (size_t)it.p * j;
+        double s = 0.0;
+        for (int r = lane; r < it.p; r += 64) s += cabs2(col[r]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) sKey[j] = (s == s) ? s : -1.0;
+    }
+    __syncthreads();
+    for (int j = tid; j < it.q; j += 1024) {
+        const double k = sKey[j];
+        int rank = 0;
+        for (int u = 0; u < it.q; ++u) {
+            const double ku = sKey[u];
+            rank += (ku > k || (ku == k && u < j)) ? 1 : 0;
+        }
+        it.perm[rank] = j;
+        it.perm_inv[j] = rank;
+    }
+    if (tid == 0) *it.count = it.q;
+}
+void launch_pivot_order(const PivotItem* d_items, int count, hipStream_t s) {
+    if (count > 0) k_pivot_order<<<count, 1024, 0, s>>>(d_items);
+}
+
 }  // namespace qh

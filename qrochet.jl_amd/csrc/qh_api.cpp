@@ -1091,6 +1091,8 @@ int qh_ctx_create(int device, qh_context** out) {
         c->device = device;
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("QH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
+        if (const char* e = std::getenv("QH_PIVOT")) c->pivot = std::atoi(e) != 0;
+        if (const char* e = std::getenv("QH_QR_PASSES")) c->jac_precondition = std::min(std::max(std::atoi(e), 0), QR_MAX_PASSES);
         // a small first chunk makes tests exercise the multi-chunk arena cheaply
         if (const char* e = std::getenv("QH_ARENA_INITIAL_BYTES")) {
             const long long v = std::atoll(e);
@@ -1266,6 +1268,11 @@ int qh_ctx_autotune(qh_context* c, int max_q, int batch, double* report, int* nc
 int qh_ctx_set_graphs(qh_context* c, int enable) {
     if (!c) return QH_ERR_ARGUMENT;
     return guard(c, [&] { c->use_graphs = enable != 0; });
+}
+
+int qh_ctx_set_pivoting(qh_context* c, int enable) {
+    if (!c) return QH_ERR_ARGUMENT;
+    return guard(c, [&] { c->pivot = enable != 0; });
 }
 
 // forget the tuned schedule (back to qh_ctx_set_jacobi / qh_ctx_set_qr_passes)

@@ -146,7 +146,14 @@ struct CompleteItem {
     const double* fro2;      // ||X||_F² before the sweeps
     double floor_rel2;
 };
-void launch_complete(const CompleteItem* d_items, int count, hipStream_t s);
+// random starts of the frozen columns, orthogonalised against the columns above the
+// floor (tile map: item, sorted column j, for every j below the item's kept bound)
+void launch_complete_project(const CompleteItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// d_counts[2i] = the first frozen sorted position of item i, d_counts[2i + 1] = its kept
+void launch_complete_count(const CompleteItem* d_items, int count, int* d_counts, hipStream_t s);
+// σ (and Λ) below 1e-280 in the frozen positions set to 0
+void launch_complete_zero_tiny(const CompleteItem* d_items, const int* d_counts, int count, hipStream_t s);
+void launch_eye_rect(cplx* p, int m, int n, hipStream_t s);
 
 // x[r, perm[j]] = f[r, j] (times σ_j if mul_sigma and σ_j > 0) for j < kept: the
 // inverse of a gather (tile map: item, column j)
@@ -179,6 +186,7 @@ struct GatherItem {
     int conj_transpose;
     int max_cols;
     int ld_out;     // non-transposed output leading dimension (0: rows)
+    const int* row_src = nullptr;  // output row c reads source row row_src[c] (nullptr: c)
 };
 // tile map: (item, tile) over 32 × 32 tiles, tile = jt * ceil(rows/32) + ct
 void launch_gather_items(const GatherItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
@@ -227,6 +235,18 @@ struct UpperHItem {
 };
 // tile map: (item, block of 256 entries of the n × n output)
 void launch_upper_h_batch(const UpperHItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// column order of the first preconditioning pass: the columns of x (p × q, ld p) by
+// decreasing norm, ties in index order; perm[k] = the column placed k-th, perm_inv its
+// inverse, *count = q (the column count a gather of the permuted copy reads)
+constexpr int PIVOT_MAXQ = 4096;
+struct PivotItem {
+    const cplx* x;
+    int p, q;
+    int* perm;
+    int* perm_inv;
+    int* count;
+};
+void launch_pivot_order(const PivotItem* d_items, int count, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // small kernels

@@ -719,6 +719,42 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     int* d_kept = reinterpret_cast<int*>(ws + off_k);
     double* d_disc = reinterpret_cast<double*>(ws + off_d);
 
+    // ---- column pivoting of the first pass: X P = Q1 R1 with the columns of X in order
+    // of decreasing norm (R1 then has a graded diagonal, and the Jacobi needs fewer
+    // sweeps).  The passes and the Jacobi then factor X P, so X's right factor is P
+    // times theirs: its output rows are gathered through P's inverse.
+    std::vector<cplx*> piv_copy(nt, nullptr);
+    std::vector<const int*> piv_inv(nt, nullptr);
+    int* piv_buf = nullptr;
+    if (c->pivot) {
+        size_t ints = 0;
+        for (const Off& o : off)
+            if (o.pre && o.q_ <= PIVOT_MAXQ) ints += 2 * (size_t)o.q_ + 1;
+        if (ints > 0) {
+            piv_buf = dalloc<int>(c, ints);
+            std::vector<PivotItem> pv;
+            std::vector<GatherItem> pg;
+            size_t at = 0;
+            for (size_t i = 0; i < nt; ++i) {
+                const Off& o = off[i];
+                if (!o.pre || o.q_ > PIVOT_MAXQ) continue;
+                int* perm = piv_buf + at;
+                int* inv = perm + o.q_;
+                int* count = inv + o.q_;
+                at += 2 * (size_t)o.q_ + 1;
+                piv_copy[i] = dalloc<cplx>(c, (size_t)o.p_ * o.q_);
+                piv_inv[i] = inv;
+                pv.push_back(PivotItem{tasks[i].x, o.p_, o.q_, perm, inv, count});
+                GatherItem g{};
+                g.src = tasks[i].x; g.ld_src = o.p_; g.rows = o.p_; g.perm = perm; g.kept = count;
+                g.rowscale_mod = 1; g.rowscale_div = 1; g.out = piv_copy[i]; g.ld_out = o.p_; g.max_cols = o.q_;
+                pg.push_back(g);
+            }
+            launch_pivot_order(upload(c, pv), (int)pv.size(), c->stream);
+            run_gather(c, pg);
+        }
+    }
+
     // ---- QR preconditioning: X = Q1 R1 and R_{j-1}^H = Q_j R_j for j = 2..npass;
     // the Jacobi runs on Y = R_npass^H (lower triangular), y[j] = R_j^H
     std::vector<std::array<QrJob, QR_MAX_PASSES + 1>> qr(nt);
@@ -729,7 +765,8 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         for (size_t i = 0; i < nt; ++i) {
             const Off& o = off[i];
             if (o.np < j) continue;
-            qr[i][j] = (j == 1) ? QrJob{tasks[i].x, o.p_, o.q_, o.p_, P(o.qv[1]), P(o.qt[1])}
+            cplx* x1 = piv_copy[i] ? piv_copy[i] : tasks[i].x;
+            qr[i][j] = (j == 1) ? QrJob{x1, o.p_, o.q_, o.p_, P(o.qv[1]), P(o.qt[1])}
                                 : QrJob{P(o.y[j - 1]), o.q_, o.q_, o.q_, P(o.qv[j]), P(o.qt[j])};
             f.push_back(qr[i][j]);
         }
@@ -745,6 +782,8 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
             }
         launch_upper_h_batch(upload(c, uh), upload(c, ut), (int)ut.size(), c->stream);
     }
+    for (cplx* t : piv_copy)
+        if (t) dfree(c, t);
 
     std::vector<JacItem> jac(nt);
     std::vector<SelectItem> sel(nt);
@@ -778,10 +817,60 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     // orthonormal completion, as LAPACK's factors are orthonormal however small σ is
     {
         std::vector<CompleteItem> comp(nt);
-        for (size_t i = 0; i < nt; ++i)
+        std::vector<int2> ct;
+        for (size_t i = 0; i < nt; ++i) {
             comp[i] = CompleteItem{jac[i].x, jac[i].p, sel[i].perm, sel[i].sigma_sorted, sel[i].lam_out,
                                    sel[i].kept, d_fro2 + i, jac[i].floor_rel2};
-        launch_complete(upload(c, comp), (int)nt, c->stream);
+            for (int j = 0; j < off[i].k_; ++j) ct.push_back(make_int2((int)i, j));
+        }
+        CompleteItem* dcomp = upload(c, comp);
+        launch_complete_project(dcomp, upload(c, ct), (int)ct.size(), c->stream);
+        int* d_cnt = dalloc<int>(c, 2 * nt);
+        launch_complete_count(dcomp, (int)nt, d_cnt, c->stream);
+        std::vector<int> cnt(2 * nt);
+        readback(c, cnt.data(), d_cnt, cnt.size() * sizeof(int));
+        // the projected starts orthonormalised among themselves: Householder QR of the
+        // frozen block (gathered to p × f), its Q written back (times σ where σ > 0)
+        std::vector<size_t> fi;
+        for (size_t i = 0; i < nt; ++i)
+            if (cnt[2 * i + 1] > cnt[2 * i]) fi.push_back(i);
+        if (!fi.empty()) {
+            launch_complete_zero_tiny(dcomp, d_cnt, (int)nt, c->stream);
+            std::vector<int> fcount(fi.size());
+            for (size_t k = 0; k < fi.size(); ++k) fcount[k] = cnt[2 * fi[k] + 1] - cnt[2 * fi[k]];
+            const int* d_f = upload(c, fcount);
+            std::vector<QrJob> jobs(fi.size());
+            std::vector<cplx*> bufs;
+            std::vector<GatherItem> gz;
+            std::vector<ScatterItem> sq;
+            std::vector<int2> st;
+            for (size_t k = 0; k < fi.size(); ++k) {
+                const size_t i = fi[k];
+                const int p = jac[i].p, f = fcount[k], first = cnt[2 * i];
+                cplx* z = dalloc<cplx>(c, (size_t)p * f);
+                cplx* v = dalloc<cplx>(c, (size_t)p * f);
+                cplx* t = dalloc<cplx>(c, (size_t)((f + QR_NB - 1) / QR_NB) * QR_NB * QR_NB);
+                cplx* q = dalloc<cplx>(c, (size_t)p * f);
+                bufs.insert(bufs.end(), {z, v, t, q});
+                GatherItem g{};
+                g.src = jac[i].x; g.ld_src = p; g.rows = p; g.perm = sel[i].perm + first; g.kept = d_f + k;
+                g.rowscale_mod = 1; g.rowscale_div = 1; g.out = z; g.ld_out = p; g.max_cols = f;
+                gz.push_back(g);
+                jobs[k] = QrJob{z, p, f, p, v, t};
+                launch_eye_rect(q, p, f, c->stream);
+                sq.push_back(ScatterItem{jac[i].x, p, sel[i].perm + first, sel[i].sigma_sorted + first, d_f + k, q,
+                                         p, 1});
+                for (int j = 0; j < f; ++j) st.push_back(make_int2((int)k, j));
+            }
+            run_gather(c, gz);
+            run_qr_factor(c, jobs);
+            std::vector<QrApplyJob> form;
+            for (size_t k = 0; k < fi.size(); ++k) form.push_back(QrApplyJob{&jobs[k], bufs[4 * k + 3], fcount[k]});
+            run_qr_apply_q(c, form);
+            launch_scatter_cols(upload(c, sq), upload(c, st), (int)st.size(), c->stream);
+            for (cplx* b : bufs) dfree(c, b);
+        }
+        dfree(c, d_cnt);
     }
     // polar refinement where asked (see SvdTask::refine).  Without preconditioning the
     // factors are the kept columns of X (divided by σ) and of V, refined in dense
@@ -905,6 +994,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
                 g.ld_src = from_x ? o.p_ : o.q_;
                 g.perm = nullptr;
                 g.divide_sigma = 0;
+                if (!from_x) g.row_src = piv_inv[i];
             } else {
                 g.src = from_x ? t.x : jac[i].v;
                 g.ld_src = from_x ? o.p_ : o.q_;
@@ -923,6 +1013,7 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         tasks[i].kept = kept[i];
         tasks[i].discarded = disc[i];
     }
+    if (piv_buf) dfree(c, piv_buf);
     dfree(c, ws);
 }
 

@@ -57,6 +57,7 @@ SIGNATURES = {
     "qh_ctx_autotune": (C.c_int, [_ctx_p, C.c_int, C.c_int, _dp, _ip]),
     "qh_ctx_autotune_reset": (C.c_int, [_ctx_p]),
     "qh_ctx_set_graphs": (C.c_int, [_ctx_p, C.c_int]),
+    "qh_ctx_set_pivoting": (C.c_int, [_ctx_p, C.c_int]),
     "qh_mps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_mps_p)]),
     "qh_pmps_create": (C.c_int, [_ctx_p, C.c_int, C.c_int, _ip, C.POINTER(_dp), C.POINTER(_mps_p)]),
     "qh_mps_rand": (C.c_int, [_ctx_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_mps_p)]),

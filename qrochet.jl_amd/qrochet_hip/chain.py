@@ -192,6 +192,11 @@ class Context:
         them one by one (`qh_ctx_set_graphs`)."""
         self.check(L.load().qh_ctx_set_graphs(self.handle, 1 if enable else 0))
 
+    def set_pivoting(self, enable: bool):
+        """First QR pass of the SVD's preconditioner on the columns ordered by decreasing
+        norm (default) or in their own order (`qh_ctx_set_pivoting`)."""
+        self.check(L.load().qh_ctx_set_pivoting(self.handle, 1 if enable else 0))
+
     def autotune_reset(self):
         self.check(L.load().qh_ctx_autotune_reset(self.handle))
 

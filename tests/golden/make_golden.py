@@ -75,6 +75,7 @@ class _SpectrumRecorder:
 CONFIG2_STEPS = 6
 CONFIG2_EXACT_STEPS = 3
 CONFIG2_LAMBDA_BONDS = (16, 32, 48)
+CONFIG2_WINDOW = 8  # singular values kept on either side of each update's cut
 
 
 def _gesvd(m):
@@ -88,7 +89,7 @@ def _config2_run(steps, svd=None):
     arrays, layers, tr, _ = wl.config2(steps=steps)
     psi = oc.Chain([a.copy() for a in arrays])
     psi0 = psi.copy()
-    kept, margin = [], []
+    kept, margin, window = [], [], []
     z_exact = None
     orig = oc._svd
     if svd is not None:
@@ -101,6 +102,11 @@ def _config2_run(steps, svd=None):
                     k = psi.bond_dims()[b - 1]
                     sv_ = rec.last
                     kept.append(k)
+                    w = np.full(2 * CONFIG2_WINDOW, np.nan)
+                    for t in range(2 * CONFIG2_WINDOW):
+                        if 0 <= k - CONFIG2_WINDOW + t < len(sv_):
+                            w[t] = sv_[k - CONFIG2_WINDOW + t]
+                    window.append(w)
                     if k < min(len(sv_), tr["maxdim"]):
                         margin.append(min(abs(np.log(sv_[k - 1] / tr["threshold"])),
                                           abs(np.log(sv_[k] / tr["threshold"]))))
@@ -112,6 +118,7 @@ def _config2_run(steps, svd=None):
         oc._svd = orig
     n = psi.n
     out = dict(kept=np.array(kept, dtype=np.int64), threshold_margin=np.array(margin), z_exact=z_exact,
+               sv_window=np.array(window),
                z=np.array([oc.expect(psi, [oc.gate1(wl.Z, i)]) for i in range(1, n + 1)]),
                norm2=np.complex128(oc.overlap(psi, psi)), overlap=np.complex128(oc.overlap(psi, psi0)),
                final_dims=np.array(psi.bond_dims(), dtype=np.int64))
@@ -127,7 +134,8 @@ def config2_saturation_outputs(steps=CONFIG2_STEPS):
     them once it has saturated.  Records, per update in schedule order, the kept
     dimension and -- where the cut-off (not the cap) decided it -- the distance of
     the decisive singular values from the cut-off, |log(σ/1e-12)| of the smallest
-    kept and the largest dropped one; all unnormalised ⟨Z_i⟩ (`expect`) after
+    kept and the largest dropped one; the singular values σ_{k-7} .. σ_{k+8} around
+    each cut k (NaN beyond the spectrum); all unnormalised ⟨Z_i⟩ (`expect`) after
     CONFIG2_EXACT_STEPS steps and at the end, Λ of three bulk bonds, ⟨ψ|ψ⟩ and the
     overlap with the initial product state.
 
@@ -140,7 +148,7 @@ def config2_saturation_outputs(steps=CONFIG2_STEPS):
     other = _config2_run(steps, _gesvd)
     out = dict(steps=np.int64(steps), exact_steps=np.int64(CONFIG2_EXACT_STEPS),
                lambda_bonds=np.array(CONFIG2_LAMBDA_BONDS, dtype=np.int64), **main)
-    for k in ("kept", "z", "overlap", "norm2") + tuple(f"lambda{b}" for b in CONFIG2_LAMBDA_BONDS):
+    for k in ("kept", "z", "overlap", "norm2", "sv_window") + tuple(f"lambda{b}" for b in CONFIG2_LAMBDA_BONDS):
         out[f"{k}_gesvd"] = other[k]
     out["z_exact_gesvd"] = other["z_exact"]
     return out

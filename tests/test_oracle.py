@@ -433,6 +433,42 @@ def test_golden_statevector_fixture():
     assert np.max(np.abs(v - fx["final_state"])) < 1e-12
 
 
+def test_saturation_fixture_is_consistent():
+    """The workload-2 saturation fixture: its first updates replayed by the oracle give
+    the recorded kept dimensions and singular-value windows; the windows agree with
+    the recorded cut-off margins; and where zgesvd's kept dimension differs from
+    zgesdd's, the values in between lie within a decade of the cut-off (the criterion
+    the GPU run is held to)."""
+    from tests.golden import make_golden
+
+    fx = np.load(make_golden.path("config2_saturation"), allow_pickle=False)
+    thr = 1e-12
+    w = make_golden.CONFIG2_WINDOW
+    win, kept, margin = fx["sv_window"], fx["kept"], fx["threshold_margin"]
+    assert win.shape == (kept.size, 2 * w)
+    fin = np.isfinite(margin)
+    assert fin.sum() > 100  # the cut-off decides many updates before the cap takes over
+    got = np.minimum(np.abs(np.log(win[fin, w - 1] / thr)), np.abs(np.log(win[fin, w] / thr)))
+    assert np.allclose(got, margin[fin], rtol=1e-12, atol=0.0)
+    assert np.all(win[fin, w - 1] > thr) and np.all(win[fin, w] <= thr)
+    alt = fx["kept_gesvd"]
+    for u in np.nonzero(alt != kept)[0]:
+        lo, hi = sorted((int(kept[u]), int(alt[u])))
+        vals = win[u, lo - (int(kept[u]) - w): hi - (int(kept[u]) - w)]
+        assert vals.size == hi - lo and np.all(np.abs(np.log10(vals / thr)) <= 1.0)
+    # replay the first 40 updates
+    arrays, layers, tr, _ = wl.config2(steps=int(fx["steps"]))
+    psi = oc.Chain([a.copy() for a in arrays])
+    u = 0
+    for layer in layers:
+        for b, g in layer:
+            if u == 40:
+                break
+            oc.evolve(psi, oc.gate2(g, b), **tr)
+            assert psi.bond_dims()[b - 1] == kept[u]
+            u += 1
+
+
 # ---------------------------------------------------------------------------
 # periodic chains (pMPS, Chain.jl:19,36-62)
 # ---------------------------------------------------------------------------
