@@ -81,7 +81,7 @@ int qh_ctx_set_timing(qh_context* ctx, int enable);
 int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_ms, int64_t* launches,
                       double* algorithmic_flops, double* algorithmic_bytes);
 int qh_ctx_reset_timing(qh_context* ctx);
-/* Jacobi tuning: block size (8, 16) and sweep limit; 0 keeps the default.  Matrices
+/* Jacobi tuning: block size (8, 16, 32) and sweep limit; 0 keeps the default.  Matrices
  * still rotating at the limit get a second window of as many sweeps before the call
  * returns QH_ERR_NOT_CONVERGED. */
 int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);

@@ -568,11 +568,333 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block width 32: a block pair is 64 columns.  Half as many rounds per sweep as block
+// width 16, so each sweep streams X and V through HBM half as often; the price is a
+// 64 × 64 eigenproblem per pair (63 inner rounds).  Eight waves; G and Q take 133 KB
+// of LDS, so one workgroup per CU.
+// ---------------------------------------------------------------------------
+// Gram tiles of one wave group (16 × 16 tiles (ti, tj), ti <= tj, of the 4 × 4 grid):
+// group 0 takes (0,0) (0,1) (0,2) (0,3) (1,1), group 1 (1,2) (1,3) (2,2) (2,3) (3,3)
+template <int GRP>
+__device__ __forceinline__ void gram64_step(const double2 (&x)[4], f64x4 (&P)[5], f64x4 (&Q)[5], f64x4 (&S)[5]) {
+    constexpr int TI[2][5] = {{0, 0, 0, 0, 1}, {1, 1, 2, 2, 3}};
+    constexpr int TJ[2][5] = {{0, 1, 2, 3, 1}, {2, 3, 2, 3, 3}};
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        const double2 a = x[TI[GRP][t]], b = x[TJ[GRP][t]];
+        // conj(a)·b by three real products: P = ar br, Q = ai bi, S = (ar - ai)(br + bi)
+        P[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, P[t], 0, 0, 0);
+        Q[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, Q[t], 0, 0, 0);
+        S[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x - a.y, b.x + b.y, S[t], 0, 0, 0);
+    }
+}
+template <int GRP>
+__device__ __forceinline__ void gram64_store(double2 (*sG)[65], const f64x4 (&P)[5], const f64x4 (&Q)[5],
+                                             const f64x4 (&S)[5], bool first, int lane) {
+    constexpr int TI[2][5] = {{0, 0, 0, 0, 1}, {1, 1, 2, 2, 3}};
+    constexpr int TJ[2][5] = {{0, 1, 2, 3, 1}, {2, 3, 2, 3, 3}};
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * TI[GRP][t] + (lane >> 4) + 4 * q, c = 16 * TJ[GRP][t] + (lane & 15);
+            const double2 g = make_double2(P[t][q] + Q[t][q], S[t][q] - P[t][q] + Q[t][q]);
+            sG[r][c] = first ? g : cadd(sG[r][c], g);
+        }
+}
+
+__global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restrict__ items,
+                                                        const int2* __restrict__ tiles, int round_arg,
+                                                        const int* __restrict__ round_base, int sort_sweeps,
+                                                        int max_inner) {
+    constexpr int BC = 64, B = 32, LP = BC + 1, CH = 32, NP = 32, NT = 512;
+    __shared__ double2 sG[BC][LP];
+    __shared__ double2 sQ[BC][LP];
+    __shared__ int sCol[BC];
+    __shared__ int sPerm[BC], sSlot[BC], sOrd[BC];
+    __shared__ double sKey[BC];
+    __shared__ int sFlag;
+    // the Gram's 32-row chunks are staged in sQ's storage (Q is initialised afterwards)
+    static_assert(BC * (CH + 1) <= BC * LP, "staging fits in sQ");
+    double2 (*sX)[CH + 1] = reinterpret_cast<double2 (*)[CH + 1]>(&sQ[0][0]);
+
+    const int tid = threadIdx.x;
+    const int2 tt = tiles[blockIdx.x];
+    const JacItem it = items[tt.x];
+    if (*((volatile int*)it.conv)) return;
+    const int round = round_base ? *round_base + round_arg : round_arg;
+    const int R = it.nb - 1;
+    const int r = round % R;
+    const double tol2 = it.tol * it.tol;
+    const int sort = round < sort_sweeps * R;
+    int bi, bj;
+    rr_pair(r, tt.y, it.nb, bi, bj);
+    if (tid < BC) {
+        const int c = (tid < B) ? bi * B + tid : bj * B + (tid - B);
+        sCol[tid] = (c < it.q) ? c : -1;
+    }
+    __syncthreads();
+    const int p = it.p;
+    const double gfloor = it.floor_rel2 * (*it.fro2);
+    const int lane = tid & 63, wv = tid >> 6;
+
+    // ---------------- Gram matrix (upper 16 × 16 tiles) ----------------
+    // Wave group grp = wv / 4 owns five tiles; wave wq = wv % 4 of the group takes rows
+    // wq + 4t (t < 8) of each 32-row chunk, four per step; the four partial sums of a
+    // tile are added in wave order.
+    {
+        const int grp = wv >> 2, wq = wv & 3;
+        f64x4 P[5], Qa[5], S[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) P[t] = Qa[t] = S[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+        constexpr int LPT = BC * CH / NT;
+        double2 pre[LPT];
+        auto fetch = [&](int row0) {
+#pragma unroll
+            for (int s2 = 0; s2 < LPT; ++s2) {
+                const int e = tid + NT * s2;
+                const int c = e / CH, rr = e % CH;
+                const int gc = sCol[c], grow = row0 + rr;
+                pre[s2] = (gc >= 0 && grow < p) ? it.x[grow + (size_t)p * gc] : cz();
+            }
+        };
+        fetch(0);
+        for (int row0 = 0; row0 < p; row0 += CH) {
+#pragma unroll
+            for (int s2 = 0; s2 < LPT; ++s2) {
+                const int e = tid + NT * s2;
+                sX[e / CH][e % CH] = pre[s2];
+            }
+            __syncthreads();
+            if (row0 + CH < p) fetch(row0 + CH);
+#pragma unroll
+            for (int s4 = 0; s4 < CH / 16; ++s4) {
+                const int rr = wq + 4 * (4 * s4 + (lane >> 4));
+                double2 x[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[t] = sX[16 * t + (lane & 15)][rr];
+                if (grp == 0) gram64_step<0>(x, P, Qa, S);
+                else gram64_step<1>(x, P, Qa, S);
+            }
+            __syncthreads();
+        }
+        for (int w = 0; w < 4; ++w) {
+            if (wq == w) {
+                if (grp == 0) gram64_store<0>(sG, P, Qa, S, w == 0, lane);
+                else gram64_store<1>(sG, P, Qa, S, w == 0, lane);
+            }
+            __syncthreads();
+        }
+    }
+    // make G exactly Hermitian (the lower tiles were not computed)
+    for (int e = tid; e < BC * BC; e += NT) {
+        const int i = e / BC, j = e % BC;
+        if (i > j) sG[i][j] = cconj(sG[j][i]);
+        else if (i == j) sG[i][i].y = 0.0;
+    }
+    __syncthreads();
+
+    // ---------------- convergence test ----------------
+    if (tid == 0) sFlag = 0;
+    __syncthreads();
+    for (int e = tid; e < BC * BC; e += NT) {
+        const int i = e / BC, j = e % BC;
+        if (i < j && needs_rotation(sG[i][i].x, sG[j][j].x, sG[i][j], tol2, gfloor)) sFlag = 1;
+    }
+    __syncthreads();
+    if (!sFlag) return;
+
+    __builtin_amdgcn_s_setprio(2);
+
+    // ---------------- EVD of G by cyclic Jacobi ----------------
+    // Thread (pb, qb), pb = tid / 32 < 16, owns the 2 × 2 blocks of G at (row pair pb,
+    // column pair qb) and (row pair pb + 16, column pair qb), and the matching rows of
+    // Q.  Both row rotations come by shuffle from the lanes of the same wave that own
+    // the diagonal blocks.  G is updated in place: one barrier after the round's reads,
+    // one after its writes; Q's rows belong to one wave each, so Q needs none.
+    for (int e = tid; e < BC * BC; e += NT) {
+        const int i = e / BC, j = e % BC;
+        sQ[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    const int inner_cap = (it.nb == 2) ? 12 : max_inner;
+    const int pb = tid / NP, qb = tid % NP;
+    const int pb2 = pb + 16;
+    const int srcA = (pb & 1) * 32 + pb, srcB = srcA + 16;
+#pragma unroll 1
+    for (int sweep = 0; sweep < inner_cap; ++sweep) {
+        if (tid == 0) sFlag = 0;
+        __syncthreads();
+        int ap, bp, ap2, bp2, aq, bq;
+        rr_pair(0, pb, BC, ap, bp);
+        rr_pair(0, pb2, BC, ap2, bp2);
+        rr_pair(0, qb, BC, aq, bq);
+#pragma unroll 1
+        for (int rnd = 0; rnd < BC - 1; ++rnd) {
+            // the blocks' G and Q entries first, so that their reads overlap the rotation
+            double2 a00 = sG[ap][aq], a01 = sG[ap][bq], a10 = sG[bp][aq], a11 = sG[bp][bq];
+            double2 b00 = sG[ap2][aq], b01 = sG[ap2][bq], b10 = sG[bp2][aq], b11 = sG[bp2][bq];
+            double2 qa[4] = {sQ[2 * pb][aq], sQ[2 * pb][bq], sQ[2 * pb + 1][aq], sQ[2 * pb + 1][bq]};
+            double2 qb2[4] = {sQ[2 * pb2][aq], sQ[2 * pb2][bq], sQ[2 * pb2 + 1][aq], sQ[2 * pb2 + 1][bq]};
+            const JRot jq = jacobi_rot(sG[aq][aq].x, sG[bq][bq].x, sG[aq][bq], tol2, gfloor);
+            JRot jpA, jpB;
+            jpA.c = __shfl(jq.c, srcA, 64);
+            jpA.sx = __shfl(jq.sx, srcA, 64);
+            jpA.sy = __shfl(jq.sy, srcA, 64);
+            jpB.c = __shfl(jq.c, srcB, 64);
+            jpB.sx = __shfl(jq.sx, srcB, 64);
+            jpB.sy = __shfl(jq.sy, srcB, 64);
+            if (pb == qb) jpA = jq;
+            if (pb2 == qb) jpB = jq;
+            __syncthreads();
+            if (rotates(jq)) {
+                rot_cols(a00, a01, jq);
+                rot_cols(a10, a11, jq);
+                rot_cols(b00, b01, jq);
+                rot_cols(b10, b11, jq);
+                rot_cols(qa[0], qa[1], jq);
+                rot_cols(qa[2], qa[3], jq);
+                rot_cols(qb2[0], qb2[1], jq);
+                rot_cols(qb2[2], qb2[3], jq);
+                sQ[2 * pb][aq] = qa[0]; sQ[2 * pb][bq] = qa[1]; sQ[2 * pb + 1][aq] = qa[2]; sQ[2 * pb + 1][bq] = qa[3];
+                sQ[2 * pb2][aq] = qb2[0]; sQ[2 * pb2][bq] = qb2[1];
+                sQ[2 * pb2 + 1][aq] = qb2[2]; sQ[2 * pb2 + 1][bq] = qb2[3];
+            }
+            if (rotates(jpA)) {
+                if (pb == qb) {
+                    a00 = make_double2(jpA.naa, 0.0);
+                    a11 = make_double2(jpA.nbb, 0.0);
+                    a01 = cz();
+                    a10 = cz();
+                } else {
+                    rot_rows(a00, a10, jpA);
+                    rot_rows(a01, a11, jpA);
+                }
+            }
+            if (rotates(jpB)) {
+                if (pb2 == qb) {
+                    b00 = make_double2(jpB.naa, 0.0);
+                    b11 = make_double2(jpB.nbb, 0.0);
+                    b01 = cz();
+                    b10 = cz();
+                } else {
+                    rot_rows(b00, b10, jpB);
+                    rot_rows(b01, b11, jpB);
+                }
+            }
+            sG[ap][aq] = a00; sG[ap][bq] = a01; sG[bp][aq] = a10; sG[bp][bq] = a11;
+            sG[ap2][aq] = b00; sG[ap2][bq] = b01; sG[bp2][aq] = b10; sG[bp2][bq] = b11;
+            if ((pb == qb && rotates(jpA)) || (pb2 == qb && rotates(jpB))) sFlag = 1;
+            rr_next(pb, BC, ap, bp);
+            rr_next(pb2, BC, ap2, bp2);
+            rr_next(qb, BC, aq, bq);
+            __syncthreads();
+        }
+        const int any = sFlag;
+        __syncthreads();
+        if (!any) break;
+        if (sweep == 0 && tid == 0) atomicAdd(it.rot, 1);
+    }
+
+    // ---------------- order eigenpairs (first sweeps only; as for block width 16) ----------------
+    if (sort) {
+        const int first = (bi < bj) ? 0 : B;
+        if (tid < BC) {
+            int o = -1;
+            double key = 0.0;
+            if (sCol[tid] >= 0) {
+                const bool in_first = tid >= first && tid < first + B;
+                int cnt = 0;
+                if (!in_first)
+                    for (int u = first; u < first + B; ++u) cnt += (sCol[u] >= 0) ? 1 : 0;
+                for (int u = in_first ? first : B - first; u < tid; ++u) cnt += (sCol[u] >= 0) ? 1 : 0;
+                o = cnt;
+                sSlot[o] = tid;
+                const double x = sG[tid][tid].x;
+                key = (x == x) ? x : -INFINITY;
+            } else {
+                sPerm[tid] = tid;
+            }
+            sOrd[tid] = o;
+            sKey[tid] = key;
+        }
+        __syncthreads();
+        if (tid < BC && sOrd[tid] >= 0) {
+            const int o = sOrd[tid];
+            const double key = sKey[tid];
+            int rank = 0;
+            for (int u = 0; u < BC; ++u) {
+                const int ou = sOrd[u];
+                const double ku = sKey[u];
+                rank += (ou >= 0 && (ku > key || (ku == key && ou < o))) ? 1 : 0;
+            }
+            sPerm[sSlot[rank]] = tid;
+        }
+        __syncthreads();
+        for (int e = tid; e < BC * BC; e += NT) {
+            const int i = e / BC, j = e % BC;
+            sG[i][j] = sQ[i][sPerm[j]];
+        }
+        __syncthreads();
+        for (int e = tid; e < BC * BC; e += NT) {
+            const int i = e / BC, j = e % BC;
+            sQ[i][j] = sG[i][j];
+        }
+        __syncthreads();
+    }
+
+    __builtin_amdgcn_s_setprio(0);
+
+    // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q  (matrix cores) ----------------
+    // 128-row panels: wave wv owns panel rows [16 wv, 16 wv + 16) times Q's four
+    // 16-column tiles; operands straight from HBM (each wave reads and writes only its
+    // own rows); three real products per complex tile: re = P1 - P2, im = P3 - P1 - P2.
+#pragma unroll 1
+    for (int mat = 0; mat < 2; ++mat) {
+        double2* base = mat ? it.v : it.x;
+        const int rows = mat ? it.q : p;
+#pragma unroll 1
+        for (int row0 = 0; row0 < rows; row0 += 128) {
+            f64x4 accP[4], accQ[4], accS[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+            const int rw = row0 + 16 * wv + (lane & 15);
+#pragma unroll 4
+            for (int st = 0; st < BC / 4; ++st) {
+                const int c = sCol[4 * st + (lane >> 4)];
+                const double2 xa = (c >= 0 && rw < rows) ? base[rw + (size_t)rows * c] : cz();
+                const double xs = xa.x + xa.y;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const double2 qv = sQ[4 * st + (lane >> 4)][16 * b + (lane & 15)];
+                    accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, qv.x, accP[b], 0, 0, 0);
+                    accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, qv.y, accQ[b], 0, 0, 0);
+                    accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qv.x + qv.y, accS[b], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int gc = sCol[16 * b + (lane & 15)];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int grow = row0 + 16 * wv + (lane >> 4) + 4 * q;
+                    if (gc >= 0 && grow < rows)
+                        base[grow + (size_t)rows * gc] =
+                            make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
+                }
+            }
+        }
+    }
+}
+
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
                          const int* round_base, int block, int sort_sweeps, int debug, int max_inner, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
         k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
+    else if (block == 32)
+        k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, max_inner);
     else
         k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
 }

@@ -1092,6 +1092,10 @@ int qh_ctx_create(int device, qh_context** out) {
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("QH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         if (const char* e = std::getenv("QH_PIVOT")) c->pivot = std::atoi(e) != 0;
+        if (const char* e = std::getenv("QH_JACOBI_BLOCK")) {
+            const int b = std::atoi(e);
+            if (b == 8 || b == 16 || b == 32) c->jac_block = b;
+        }
         if (const char* e = std::getenv("QH_QR_PASSES")) c->jac_precondition = std::min(std::max(std::atoi(e), 0), QR_MAX_PASSES);
         // a small first chunk makes tests exercise the multi-chunk arena cheaply
         if (const char* e = std::getenv("QH_ARENA_INITIAL_BYTES")) {
@@ -1168,7 +1172,8 @@ int qh_ctx_reset_timing(qh_context* c) {
 int qh_ctx_set_jacobi(qh_context* c, int block, int max_sweeps) {
     if (!c) return QH_ERR_ARGUMENT;
     return guard(c, [&] {
-        if (block != 0 && block != 8 && block != 16) throw Error(QH_ERR_ARGUMENT, "Jacobi block must be 8 or 16");
+        if (block != 0 && block != 8 && block != 16 && block != 32)
+            throw Error(QH_ERR_ARGUMENT, "Jacobi block must be 8, 16 or 32");
         if (block) {
             c->jac_block = block;
             c->tuned_block.clear();  // an explicit setting replaces the tuned one
@@ -1189,10 +1194,12 @@ int qh_ctx_set_qr_passes(qh_context* c, int passes) {
 
 // The SVD schedule tuner.  For each size class q = 64, 128, ... <= max_q it factors a
 // batch of q × q synthetic matrices -- half with i.i.d. uniform entries (a flat
-// spectrum, like Haar-random gates on full-rank states), half with their columns
-// scaled from 1 down to 1e-12 (a decaying spectrum, like a truncated time evolution)
+// spectrum, like Haar-random gates on full-rank states), half products A·D·B of
+// uniform matrices with D decaying from 1 to 1e-12 (a graded spectrum whose singular
+// vectors are not aligned with the columns, like a truncated time evolution)
 // -- through the batched SVD under each candidate schedule (Jacobi block width 8 or
-// 16, 1..4 QR preconditioning passes), timed end to end after one warm-up run, and
+// 16, 1..4 QR preconditioning passes; width 32 is left out: a batch runs at its
+// largest class's width, and width 32 slows the small matrices of mixed batches), timed end to end after one warm-up run, and
 // keeps the fastest for the class.  Batches shrink with q² above q = 512 so every
 // class costs about the same.  The candidates change the rounding of the results,
 // never their meaning.
@@ -1209,16 +1216,29 @@ int qh_ctx_autotune(qh_context* c, int max_q, int batch, double* report, int* nc
         for (int q = 64; q <= max_q; q *= 2, ++ncls) {
             const int b = std::max(2, q <= 512 ? batch : (int)((long long)batch * 512 * 512 / ((long long)q * q)));
             const size_t elems = (size_t)q * q;
-            std::vector<cplx*> src(b), work(b);
+            std::vector<cplx*> src(b), work(b), factors;
             std::vector<double*> lam(b);
+            std::vector<GemmItem> prod;
             for (int i = 0; i < b; ++i) {
                 src[i] = dalloc<cplx>(c, elems);
                 work[i] = dalloc<cplx>(c, elems);
                 lam[i] = dalloc<double>(c, q);
-                launch_rand_uniform(src[i], elems, 0x5eedull + (unsigned long long)q, (unsigned long long)i, 1,
-                                    c->stream);
-                if (i % 2) launch_scale_cols_geometric(src[i], q, q, 12.0, c->stream);
+                const unsigned long long seed = 0x5eedull + (unsigned long long)q;
+                if (i % 2 == 0) {
+                    launch_rand_uniform(src[i], elems, seed, (unsigned long long)i, 1, c->stream);
+                } else {  // A diag(10^(-12 j/(q-1))) B: a graded spectrum in no particular basis
+                    cplx* fa = dalloc<cplx>(c, elems);
+                    cplx* fb = dalloc<cplx>(c, elems);
+                    factors.push_back(fa);
+                    factors.push_back(fb);
+                    launch_rand_uniform(fa, elems, seed, 2ull * i, 1, c->stream);
+                    launch_rand_uniform(fb, elems, seed, 2ull * i + 1, 1, c->stream);
+                    launch_scale_cols_geometric(fa, q, q, 12.0, c->stream);
+                    prod.push_back(gemm_item(fa, GEMM_N, q, fb, GEMM_N, q, src[i], q, q, q, q));
+                }
             }
+            run_gemm(c, prod, QH_PREC_FP64);
+            for (cplx* f : factors) dfree(c, f);
             auto run = [&](int block, int passes) {
                 c->jac_block = block;
                 c->jac_precondition = passes;

@@ -681,7 +681,10 @@ void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
         o.q_ = std::min(t.mr, t.mc);
         o.k_ = t.kmax();
         const size_t p = o.p_, q = o.q_;
-        o.np = (int)q > 2 * bw ? qr_passes_for(c, (int)q) : 0;
+        // QR preconditioning above one block pair of the narrower widths (q > 16 or
+        // 32); the Jacobi's relative accuracy on row-graded θ needs it however wide
+        // the blocks are
+        o.np = (int)q > std::min(2 * bw, 32) ? qr_passes_for(c, (int)q) : 0;
         o.pre = o.np > 0;
         const int npass = o.np;
         o.v = take(q * q * sizeof(cplx));

@@ -582,12 +582,12 @@ def test_qr_preconditioning_passes(qh, passes):
         ctx.set_qr_passes(5)
 
 
-@pytest.mark.parametrize("block", [8, 16])
+@pytest.mark.parametrize("block", [8, 16, 32])
 def test_jacobi_block_sizes(qh, block):
-    """Both column-block widths of the Jacobi tournament (8 and 16 columns per block,
-    `qh_ctx_set_jacobi`) evolve like the dense state vector: untruncated brickwork
-    layers on 12 sites (theta up to 128 x 128: 16 blocks of 8 or 8 blocks of 16,
-    QR-preconditioned, and the small thetas at the chain ends with one block pair)."""
+    """Every column-block width of the Jacobi tournament (8, 16 and 32 columns per
+    block, `qh_ctx_set_jacobi`) evolves like the dense state vector: untruncated
+    brickwork layers on 12 sites (theta up to 128 x 128: 16 blocks of 8, 8 of 16 or 4 of
+    32, QR-preconditioned, and the small thetas at the chain ends with one block pair)."""
     rng = np.random.default_rng(23)
     n = 12
     arrays = wl.left_canonize_normalize(wl.random_mps_arrays(rng, n, 2, wl.max_profile(n, 2, 32)))
