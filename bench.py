@@ -533,12 +533,16 @@ def main():
         extra["saturated_plain_fp64_updates_per_s"] = round(sample_updates / tp, 2)
         extra["batched_over_plain"] = round(tp / tb, 3)
         # the mixed-precision target: batched FP32-contraction updates against the plain
-        # (per-call, unbatched) FP64 form, at <= 1e-6 error in <Z_i>
+        # (per-call, unbatched) FP64 form, at <= 1e-6 error in <Z_i>; and, separately,
+        # against the batched FP64 form (the SVD is FP64 in both, so this ratio is ~1)
         m32 = mixed["fp32"]
         extra["mixed_precision_target"] = {
             "mixed_over_plain_fp64": round(m32["updates_per_s"] * tp / sample_updates, 3),
+            "mixed_over_batched_fp64": round(m32["updates_per_s"] * tb / sample_updates, 3),
             "max_abs_dZ_vs_fp64": m32["max_abs_dZ_vs_fp64"], "bound": 1e-6, "speedup_goal": 5.0,
-            "met": bool(m32["updates_per_s"] * tp / sample_updates >= 5.0 and m32["max_abs_dZ_vs_fp64"] <= 1e-6)}
+            "met": bool(m32["updates_per_s"] * tp / sample_updates >= 5.0 and m32["max_abs_dZ_vs_fp64"] <= 1e-6),
+            "note": "speed from batching; the SVD stays FP64 (FP32 Jacobi sweeps cannot resolve the "
+                    "spectrum below ~1e-7 of sigma_1, where half the kept values of a saturated theta lie)"}
         extra["saturated_sample"] = f"{args.sample_layers} layers ({sample_updates} updates) at chi<=256 after 10 Trotter steps"
 
     log("plain sample done")

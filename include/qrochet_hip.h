@@ -81,7 +81,8 @@ int qh_ctx_set_timing(qh_context* ctx, int enable);
 int qh_ctx_get_timing(qh_context* ctx, const char* kernel_class, double* total_ms, int64_t* launches,
                       double* algorithmic_flops, double* algorithmic_bytes);
 int qh_ctx_reset_timing(qh_context* ctx);
-/* Jacobi tuning: block size (8, 16, 32) and sweep limit; 0 keeps the default.  Matrices
+/* Jacobi tuning: block size (8, 16, 32) and sweep limit; 0 keeps the current setting
+ * (by default automatic: 16, and 32 for batches with a matrix of >= 2048 columns).  Matrices
  * still rotating at the limit get a second window of as many sweeps before the call
  * returns QH_ERR_NOT_CONVERGED. */
 int qh_ctx_set_jacobi(qh_context* ctx, int block, int max_sweeps);
@@ -102,9 +103,9 @@ int qh_ctx_set_pivoting(qh_context* ctx, int enable);
 /* Schedule tuner of the batched SVD (no reference counterpart; the project's
  * autotuner): for each size class q = 64, 128, ... <= max_q (at most 4096) it times
  * the batched SVD of `batch` synthetic q × q matrices (flat and decaying spectra;
- * fewer above q = 512) under every candidate schedule -- Jacobi block width 8 or 16
- * and 1..4 QR preconditioning passes -- and keeps the fastest per class for this
- * context: a matrix then gets the passes of the smallest class >= its column count,
+ * fewer above q = 512) under the context's current schedule and every candidate --
+ * Jacobi block width 8 or 16 and 1..4 QR preconditioning passes -- and keeps the
+ * fastest per class for this context: a matrix then gets the passes of the smallest class >= its column count,
  * and a batch the block width of its largest member's class.  report (may be NULL):
  * 5 doubles per class (q, block, passes, best ms, ms of the previous setting), at most
  * 8 classes; nclasses (may be NULL) receives their number.  qh_ctx_set_jacobi and

@@ -1257,12 +1257,14 @@ int qh_ctx_autotune(qh_context* c, int max_q, int batch, double* report, int* nc
                 return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             };
             (void)run(16, 3);  // warm-up: allocations of the pool, first launches
-            double best = 1e300, dflt = 0.0;
-            int bb = 16, bp = 3;
+            // the schedule the context would use without the tuner
+            const int dflt_block = keep_block > 0 ? keep_block : (q >= 2048 ? 32 : 16);
+            const double dflt = run(dflt_block, keep_passes);
+            double best = dflt;
+            int bb = dflt_block, bp = keep_passes;
             for (int block : {8, 16})
                 for (int passes = 1; passes <= QR_MAX_PASSES; ++passes) {
                     const double ms = run(block, passes);
-                    if (block == keep_block && passes == keep_passes) dflt = ms;
                     if (ms < best) { best = ms; bb = block; bp = passes; }
                 }
             passes_tab[q] = bp;

@@ -318,7 +318,7 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
 // ---------------------------------------------------------------------------
 void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, int block) {
     if (items.empty()) return;
-    if (block <= 0) block = c->jac_block;
+    if (block <= 0) block = c->jac_block > 0 ? c->jac_block : 16;
     const int bc = 2 * block;
     std::vector<int2> id_tiles, tiles;
     int rmax = 1;
@@ -630,6 +630,18 @@ void polar_refine(qh_context* c, const std::vector<PolarJob>& jobs) {
 
 // QR preconditioning passes for a matrix with q columns: the tuned value of its size
 // class (qh_ctx_autotune) if there is one, else the context's setting
+// the tuned width of the batch's largest size class, else the context's setting,
+// else the automatic choice
+int block_for(const qh_context* c, int qmax) {
+    if (!c->tuned_block.empty()) {
+        auto it = c->tuned_block.lower_bound(qmax);
+        if (it == c->tuned_block.end()) --it;
+        return it->second;
+    }
+    if (c->jac_block > 0) return c->jac_block;
+    return qmax >= 2048 ? 32 : 16;
+}
+
 int qr_passes_for(const qh_context* c, int q) {
     int passes = c->jac_precondition;
     if (!c->tuned_passes.empty()) {
@@ -649,16 +661,9 @@ int SvdTask::kmax() const {
 void run_svd_tasks(qh_context* c, std::vector<SvdTask>& tasks) {
     if (tasks.empty()) return;
     const size_t nt = tasks.size();
-    // Jacobi block width of this batch: the tuned one of its largest size class, else
-    // the context's setting
-    int bw = c->jac_block;
-    if (!c->tuned_block.empty()) {
-        int qmax = 0;
-        for (const SvdTask& t : tasks) qmax = std::max(qmax, std::min(t.mr, t.mc));
-        auto it = c->tuned_block.lower_bound(qmax);
-        if (it == c->tuned_block.end()) --it;
-        bw = it->second;
-    }
+    int qmax = 0;
+    for (const SvdTask& t : tasks) qmax = std::max(qmax, std::min(t.mr, t.mc));
+    const int bw = block_for(c, qmax);
     // workspace layout per task
     size_t bytes = 0;
     auto take = [&](size_t b) {
