@@ -175,6 +175,10 @@ end
 set_graphs!(context::Context, enable::Bool) =
     check(context, ccall((:qh_ctx_set_graphs, libqh), Cint, (Ptr{Cvoid}, Cint), context.ptr, enable ? 1 : 0))
 
+"First QR pass of the SVD's preconditioner on the columns ordered by decreasing norm (`true`, default) or as they are."
+set_pivoting!(context::Context, enable::Bool) =
+    check(context, ccall((:qh_ctx_set_pivoting, libqh), Cint, (Ptr{Cvoid}, Cint), context.ptr, enable ? 1 : 0))
+
 "`overlap(a, b)` without device allocation (`qh_overlap_noalloc`; open chains up to about χ = 1024)."
 function overlap_noalloc(a::HIPChain, b::HIPChain)
     out = zeros(Float64, 2)
@@ -348,6 +352,6 @@ for f in (:isleftcanonical, :isrightcanonical)
     end
 end
 
-export HIPChain, simple_update!, evolve_layer!, autotune!, set_graphs!, overlap_noalloc
+export HIPChain, simple_update!, evolve_layer!, autotune!, set_graphs!, set_pivoting!, overlap_noalloc
 
 end
