@@ -4,7 +4,8 @@ P times the one the passes and the Jacobi produce.  Checked against the oracle w
 1, 2 and 3 QR passes (the permutation lands on either factor of the Jacobi as the
 pass count is odd or even), on untruncated evolution (every θ of 32..128 columns goes
 through the preconditioner; tall and wide θ both occur) and on a truncated batched
-layer, with pivoting on and off."""
+layer, with pivoting on and off; and the automatic block width 32 on a θ of 2048
+columns against LAPACK."""
 import numpy as np
 import pytest
 
@@ -62,3 +63,31 @@ def test_truncated_layer_matches_oracle_with_and_without_pivoting(qh, passes):
         assert psi.bond_dims() == ref.bond_dims()
         z = qh.expect_many(psi, [[qh.gate(wl.Z, i)] for i in range(1, n + 1)])
         assert np.max(np.abs(z - z_ref)) < 1e-10
+
+
+def test_width_32_on_a_2048_column_theta(qh):
+    """A batch whose widest θ has 2048 or more columns runs the Jacobi at block width 32
+    by default (64-column pairs, with the pivoted preconditioner): one 2048 × 2048 θ
+    (a gate on two sites between bonds of 1024) against LAPACK's singular values, and
+    both new site tensors isometric."""
+    rng = np.random.default_rng(97)
+    chi = 1024
+    arrays = wl.random_mps_arrays(rng, 4, 2, [chi, chi, chi])
+    g = wl.haar_gate4(rng)
+    th = oc.theta(oc.Chain([a.copy() for a in arrays]), 2, g)
+    cl, d, _, cr = th.shape
+    s_ref = np.linalg.svd(th.reshape(cl * d, d * cr), compute_uv=False)[:chi]
+    psi = qh.from_internal(arrays)
+    info = {}
+    qh.evolve_(psi, qh.gate(g, 2, 3), maxdim=chi, info=info)
+    assert info["kept"] == chi
+    lam = psi.lambda_(2)
+    err = np.max(np.abs(lam - s_ref)) / s_ref[0]
+    a2, a3 = psi.internal_arrays()[1:3]
+    u = a2.reshape(-1, a2.shape[2])
+    vh = a3.reshape(a3.shape[0], -1)
+    iso_u = np.max(np.abs(u.conj().T @ u - np.eye(chi)))
+    iso_v = np.max(np.abs(vh @ vh.conj().T - np.eye(chi)))
+    print(f"2048 x 2048: max |sigma - sigma_lapack| / sigma_1 = {err:.2e}, isometry {iso_u:.1e} / {iso_v:.1e}")
+    assert err < 1e-12
+    assert iso_u < 1e-11 and iso_v < 1e-11
