@@ -584,9 +584,9 @@ __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__
     }
 }
 
-// Look-ahead step of the factorisation: workgroups [0, count) factor panel j0 + QR_NB
-// of every item (its columns already carry panel j0's update), the others apply
-// panel j0's block reflector to the trailing tiles beyond that panel.  The two
+// Look-ahead step of the factorisation: workgroup i < count applies panel j0's block
+// reflector to the next panel's columns of item i and then factors that panel; the
+// others apply panel j0's block reflector to the trailing tiles beyond it.  The two
 // parts touch disjoint columns and disjoint reflector storage, so they run in one
 // launch and the panel's latency hides behind the trailing update.
 // (two waves per SIMD: the trailing-update workgroups need the occupancy)
@@ -602,6 +602,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const QrItem it = items[blockIdx.x];
         const int jn = j0 + QR_NB;
         if (jn >= it.n) return;
+        const QrApplyItem ai = app[blockIdx.x];  // the same item's reflectors
+        qr_apply_panel(ai, jn, min(QR_TC, ai.ncols - jn), j0, QR_NB, 1, sh.a);
+        __syncthreads();
         qr_panel_regs<256>(it, jn, sh.p);  // at most 512 rows (host routing)
         return;
     }

@@ -571,13 +571,12 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
         }
         return;
     }
-    // look-ahead: after panel j0, update only the next panel's columns, then factor
-    // that panel in the same launch as the rest of panel j0's trailing update
+    // look-ahead: after panel j0, one launch updates the next panel's columns and
+    // factors that panel (a workgroup per item) while the others apply panel j0 to the
+    // columns beyond it
     launch_qr_panel(di, (int)items.size(), 0, c->stream);
-    for (int j0 = 0; j0 + QR_NB < nmax; j0 += QR_NB) {
-        launch_qr_apply(da, dn, (int)next_tiles.size(), j0, 1, c->stream);
+    for (int j0 = 0; j0 + QR_NB < nmax; j0 += QR_NB)
         launch_qr_step(di, (int)items.size(), da, dr, (int)rest_tiles.size(), j0, c->stream);
-    }
 }
 
 void run_qr_apply_q(qh_context* c, const std::vector<QrApplyJob>& jobs) {
