@@ -208,6 +208,26 @@ __device__ __forceinline__ void gemm_tile_f64(const GemmItem& g, int tile) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_gemm_reduce(const GemmReduceItem* __restrict__ items,
+                                                     const int2* __restrict__ tiles) {
+    const int2 tt = tiles[blockIdx.x];
+    const GemmReduceItem g = items[tt.x];
+    const size_t mn = (size_t)g.m * g.n;
+    const size_t e = (size_t)tt.y * 256 + threadIdx.x;
+    if (e >= mn) return;
+    double2 v = g.w[e];
+    for (int s = 1; s < g.slices; ++s) v = cadd(v, g.w[e + mn * s]);
+    const int r = (int)(e % g.m), c = (int)(e / g.m);
+    v = cmul(g.alpha, v);
+    v = cscale(v, (g.rowscale ? g.rowscale[r] : 1.0) * (g.colscale ? g.colscale[c] : 1.0));
+    double2* dst = g.c + r + (size_t)g.ldc * c;
+    if (g.beta.x != 0.0 || g.beta.y != 0.0) v = cfma(g.beta, *dst, v);
+    *dst = v;
+}
+void launch_gemm_reduce(const GemmReduceItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
+    if (ntiles > 0) k_gemm_reduce<<<ntiles, 256, 0, s>>>(d_items, d_tiles);
+}
+
 __global__ __launch_bounds__(256) void k_gemm_f64(const GemmItem* __restrict__ items, const int2* __restrict__ tiles) {
     const int2 tt = tiles[blockIdx.x];
     const GemmItem g = items[tt.x];

@@ -71,6 +71,19 @@ __device__ __forceinline__ cplx load_op(const cplx* x, int ld, int r, int c, int
 }
 // FP64 on the vector ALUs
 void launch_gemm(const GemmItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
+// split-K epilogue: C = alpha (Σ_s W_s) rowscale colscale (+ beta C), the slices W_s
+// (m × n each, ld m, consecutive) added in order s = 0, 1, ...
+struct GemmReduceItem {
+    const cplx* w;
+    int slices;
+    cplx* c;
+    int m, n, ldc;
+    cplx alpha, beta;
+    const double* rowscale;
+    const double* colscale;
+};
+// tile map: (item, block of 256 entries of the m × n output)
+void launch_gemm_reduce(const GemmReduceItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // the same kernel for one GEMM passed by value (tiles and divisors filled in here)
 void launch_gemm_arg(GemmItem g, hipStream_t s);
 // FP32 or bf16x3 products on MFMA with FP32 accumulation (precision = QH_PREC_FP32 /

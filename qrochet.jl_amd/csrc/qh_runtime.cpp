@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 
 namespace qh {
 
@@ -291,7 +292,78 @@ GemmItem gemm_item(const cplx* a, int opa, int lda, const cplx* b, int opb, int 
     return g;
 }
 
+// Split-K for an FP64 product whose output tiles alone leave most CUs idle (the
+// environment steps of a sweep: a χ × χ output from a contraction over dχ): the K range
+// is cut into slices computed as separate partial products into a workspace, then
+// added in slice order.  The number of slices depends on the product's own shape only,
+// so a product gets the same rounding alone and inside any batch.
+static int gemm_slices(const GemmItem& g) {
+    const int tiles = ((g.m + GEMM_TM - 1) / GEMM_TM) * ((g.n + GEMM_TN - 1) / GEMM_TN);
+    if (tiles >= 64 || g.k < 256) return 1;
+    return std::max(1, std::min({8, 256 / tiles, g.k / 64}));
+}
+
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
+    if (items.empty()) return;
+    if (precision == QH_PREC_FP64) {
+        // slice length per product: whole K tiles, and whole periods of the K scaling
+        std::vector<int> ks(items.size(), 0), slices(items.size(), 1);
+        size_t ws_elems = 0;
+        for (size_t i = 0; i < items.size(); ++i) {
+            const GemmItem& g = items[i];
+            const int s0 = gemm_slices(g);
+            if (s0 <= 1) continue;
+            int align = GEMM_TK;
+            if (g.dk) align = std::lcm(align, std::max(g.dk_div, 1) * std::max(g.dk_mod, 1));
+            ks[i] = ((g.k + s0 - 1) / s0 + align - 1) / align * align;
+            slices[i] = (g.k + ks[i] - 1) / ks[i];
+            if (slices[i] > 1) ws_elems += (size_t)slices[i] * g.m * g.n;
+        }
+        if (ws_elems > 0) {
+            cplx* ws = dalloc<cplx>(c, ws_elems);
+            std::vector<GemmItem> parts;
+            std::vector<GemmReduceItem> red;
+            size_t at = 0;
+            for (size_t i = 0; i < items.size(); ++i) {
+                const GemmItem& g = items[i];
+                if (slices[i] <= 1) {
+                    parts.push_back(g);
+                    continue;
+                }
+                const size_t mn = (size_t)g.m * g.n;
+                red.push_back(GemmReduceItem{ws + at, slices[i], g.c, g.m, g.n, g.ldc, g.alpha, g.beta, g.rowscale,
+                                             g.colscale});
+                for (int s = 0; s < slices[i]; ++s) {
+                    const int k0 = s * ks[i];
+                    GemmItem p = g;
+                    p.k = std::min(ks[i], g.k - k0);
+                    p.a = (g.opa == GEMM_N || g.opa == GEMM_CONJ) ? g.a + (size_t)g.lda * k0 : g.a + k0;
+                    p.b = (g.opb == GEMM_N || g.opb == GEMM_CONJ) ? g.b + k0 : g.b + (size_t)g.ldb * k0;
+                    p.c = ws + at + s * mn;
+                    p.ldc = g.m;
+                    p.alpha = make_double2(1.0, 0.0);
+                    p.beta = make_double2(0.0, 0.0);
+                    p.rowscale = nullptr;
+                    p.colscale = nullptr;
+                    parts.push_back(p);
+                }
+                at += (size_t)slices[i] * mn;
+            }
+            run_gemm_tiles(c, parts);
+            std::vector<int2> rt;
+            for (size_t i = 0; i < red.size(); ++i) {
+                const size_t mn = (size_t)red[i].m * red[i].n;
+                for (size_t blk = 0; blk * 256 < mn; ++blk) rt.push_back(make_int2((int)i, (int)blk));
+            }
+            launch_gemm_reduce(upload(c, red), upload(c, rt), (int)rt.size(), c->stream);
+            dfree(c, ws);
+            return;
+        }
+    }
+    run_gemm_tiles(c, items, precision);
+}
+
+void run_gemm_tiles(qh_context* c, std::vector<GemmItem>& items, int precision) {
     if (items.empty()) return;
     std::vector<int2> tiles;
     double flops = 0, bytes = 0;

@@ -155,6 +155,8 @@ void noalloc_setup(int device);
 
 // ---- batched building blocks ----
 void run_theta(qh_context* c, std::vector<ThetaItem>& items, int precision = QH_PREC_FP64);
+// the GEMM launch itself (no split-K)
+void run_gemm_tiles(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision = QH_PREC_FP64);
 // runs set-identity + sweeps until every item converged; throws on sweep limit.
 // fro2_out (optional, one double per item) receives ||X||_F² and stays valid after.
