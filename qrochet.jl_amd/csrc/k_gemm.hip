@@ -159,7 +159,7 @@ __device__ __forceinline__ void gemm_tile_f64(const GemmItem& g, int tile) {
             double2 v = cz();
             if (gr < g.m && gk < g.k) {
                 v = load_op(g.a, g.lda, gr, gk, g.opa);
-                if (g.dk) v = cscale(v, g.dk[(gk / g.dk_div) % g.dk_mod]);
+                if (g.dk) v = cscale(v, g.dk[((gk + g.dk_off) / g.dk_div) % g.dk_mod]);
             }
             sA[kk][row] = v;
         }

@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const GemmItem* __restrict__ 
             double2 v = cz();
             if (gr < g.m && gk < g.k) {
                 v = load_op(g.a, g.lda, gr, gk, g.opa);
-                if (g.dk) v = cscale(v, g.dk[(gk / g.dk_div) % g.dk_mod]);
+                if (g.dk) v = cscale(v, g.dk[((gk + g.dk_off) / g.dk_div) % g.dk_mod]);
             }
             sA[kk][row] = make_float2((float)v.x, (float)v.y);
         }

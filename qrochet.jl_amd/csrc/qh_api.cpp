@@ -427,7 +427,10 @@ void canonize_core(qh_context* c, qh_mps* m) {
         std::vector<UpperHItem> uh{UpperHItem{M, rows, cols, RH[i]}};
         std::vector<int2> ut;
         for (size_t b = 0; b * 256 < (size_t)cols * cols; ++b) ut.push_back(make_int2(0, (int)b));
-        launch_upper_h_batch(upload(c, uh), upload(c, ut), (int)ut.size(), c->stream);
+        UpperHItem* duh;
+        int2* dut;
+        upload2(c, uh, ut, duh, dut);
+        launch_upper_h_batch(duh, dut, (int)ut.size(), c->stream);
         // M_{i+1} = R_i B_{i+1}  (χ_i × d χ_{i+1}); R_i = (R_i^H)^H
         const int n2 = d * m->chi[i + 1];
         cplx* nxt = dalloc<cplx>(c, (size_t)cols * n2);
@@ -982,7 +985,12 @@ void expect_terms_batched(qh_context* c, const qh_mps* m, const std::vector<std:
                     curl[q] = rb;
                 }
             }
-            if (!ap.empty()) launch_apply_op_batch(upload(c, ap), upload(c, ap_tiles), (int)ap_tiles.size(), c->stream);
+            if (!ap.empty()) {
+                ApplyItem* dap;
+                int2* dapt;
+                upload2(c, ap, ap_tiles, dap, dapt);
+                launch_apply_op_batch(dap, dapt, (int)ap_tiles.size(), c->stream);
+            }
             run_gemm(c, gm);
             for (cplx* p : to_free) dfree(c, p);
         }

@@ -49,7 +49,7 @@ struct GemmItem {
     const cplx* a;
     const cplx* b;
     cplx* c;
-    const double* dk;        // K scaling, index (k / dk_div) % dk_mod (null = none)
+    const double* dk;        // K scaling, index ((k + dk_off) / dk_div) % dk_mod (null = none)
     const double* rowscale;  // M
     const double* colscale;  // N
     cplx alpha, beta;
@@ -57,6 +57,7 @@ struct GemmItem {
     int opa, opb;            // GEMM_N, GEMM_C (conjugate transpose), GEMM_T (transpose), GEMM_CONJ
     int dk_mod, dk_div;
     int tiles_m, tiles_n;
+    int dk_off;              // K offset of the scaling index (a K-slice of a larger product)
 };
 enum { GEMM_N = 0, GEMM_C = 1, GEMM_T = 2, GEMM_CONJ = 3 };
 // element (r, c) of op(X) where X is stored column-major with leading dimension ld

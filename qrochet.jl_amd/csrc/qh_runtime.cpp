@@ -9,7 +9,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <numeric>
 
 namespace qh {
 
@@ -161,6 +160,34 @@ void release_call_allocs(qh_context* c) {
     c->call_allocs.clear();
 }
 
+void arena_upload_segments(qh_context* c, int count, const void* const* srcs, const size_t* bytes, void** outs) {
+    size_t need = 0;
+    for (int i = 0; i < count; ++i) need += (bytes[i] + 255) & ~size_t(255);
+    if (c->arena.empty() || c->arena_off + need > c->arena.back().cap) {
+        qh_context::ArenaChunk ch;
+        ch.cap = std::max(need, c->arena.empty() ? c->arena_first : c->arena.back().cap * 2);
+        QH_HIP(hipHostMalloc((void**)&ch.h, ch.cap, hipHostMallocDefault));
+        if (hipMalloc((void**)&ch.d, ch.cap) != hipSuccess) {
+            (void)hipHostFree(ch.h);
+            throw Error(QH_ERR_OOM, "descriptor arena allocation failed");
+        }
+        c->arena.push_back(ch);
+        c->arena_off = 0;
+    }
+    char* h = c->arena.back().h + c->arena_off;
+    char* d = c->arena.back().d + c->arena_off;
+    size_t at = 0, last = 0;
+    for (int i = 0; i < count; ++i) {
+        if (bytes[i]) std::memcpy(h + at, srcs[i], bytes[i]);
+        outs[i] = d + at;
+        last = at + bytes[i];
+        at += (bytes[i] + 255) & ~size_t(255);
+    }
+    // one copy for all segments (the alignment gaps travel along)
+    if (last) QH_HIP(hipMemcpyAsync(d, h, last, hipMemcpyHostToDevice, c->stream));
+    c->arena_off += need;
+}
+
 void* arena_upload(qh_context* c, const void* src, size_t bytes) {
     const size_t need = (bytes + 255) & ~size_t(255);
     if (c->arena.empty() || c->arena_off + need > c->arena.back().cap) {
@@ -306,16 +333,14 @@ static int gemm_slices(const GemmItem& g) {
 void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
     if (items.empty()) return;
     if (precision == QH_PREC_FP64) {
-        // slice length per product: whole K tiles, and whole periods of the K scaling
+        // slice length per product: whole K tiles (a slice's K scaling index is offset)
         std::vector<int> ks(items.size(), 0), slices(items.size(), 1);
         size_t ws_elems = 0;
         for (size_t i = 0; i < items.size(); ++i) {
             const GemmItem& g = items[i];
             const int s0 = gemm_slices(g);
             if (s0 <= 1) continue;
-            int align = GEMM_TK;
-            if (g.dk) align = std::lcm(align, std::max(g.dk_div, 1) * std::max(g.dk_mod, 1));
-            ks[i] = ((g.k + s0 - 1) / s0 + align - 1) / align * align;
+            ks[i] = ((g.k + s0 - 1) / s0 + GEMM_TK - 1) / GEMM_TK * GEMM_TK;
             slices[i] = (g.k + ks[i] - 1) / ks[i];
             if (slices[i] > 1) ws_elems += (size_t)slices[i] * g.m * g.n;
         }
@@ -337,6 +362,7 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
                     const int k0 = s * ks[i];
                     GemmItem p = g;
                     p.k = std::min(ks[i], g.k - k0);
+                    p.dk_off = g.dk_off + k0;
                     p.a = (g.opa == GEMM_N || g.opa == GEMM_CONJ) ? g.a + (size_t)g.lda * k0 : g.a + k0;
                     p.b = (g.opb == GEMM_N || g.opb == GEMM_CONJ) ? g.b + k0 : g.b + (size_t)g.ldb * k0;
                     p.c = ws + at + s * mn;
@@ -355,7 +381,10 @@ void run_gemm(qh_context* c, std::vector<GemmItem>& items, int precision) {
                 const size_t mn = (size_t)red[i].m * red[i].n;
                 for (size_t blk = 0; blk * 256 < mn; ++blk) rt.push_back(make_int2((int)i, (int)blk));
             }
-            launch_gemm_reduce(upload(c, red), upload(c, rt), (int)rt.size(), c->stream);
+            GemmReduceItem* dr;
+            int2* drt;
+            upload2(c, red, rt, dr, drt);
+            launch_gemm_reduce(dr, drt, (int)rt.size(), c->stream);
             dfree(c, ws);
             return;
         }
@@ -378,8 +407,9 @@ void run_gemm_tiles(qh_context* c, std::vector<GemmItem>& items, int precision) 
         bytes += 16.0 * ((double)g.m * g.k + (double)g.k * g.n + (double)g.m * g.n);
     }
     if (tiles.empty()) return;
-    GemmItem* di = upload(c, items);
-    int2* dt = upload(c, tiles);
+    GemmItem* di;
+    int2* dt;
+    upload2(c, items, tiles, di, dt);
     Timed tm(c, "gemm", flops, bytes);
     if (precision == QH_PREC_FP64) launch_gemm(di, dt, (int)tiles.size(), c->stream);
     else launch_gemm_mfma(di, dt, (int)tiles.size(), precision, c->stream);
@@ -597,8 +627,9 @@ void run_gather(qh_context* c, std::vector<GatherItem>& items) {
         bytes += 32.0 * items[i].rows * (double)items[i].max_cols;
     }
     if (tiles.empty()) return;
-    GatherItem* di = upload(c, items);
-    int2* dt = upload(c, tiles);
+    GatherItem* di;
+    int2* dt;
+    upload2(c, items, tiles, di, dt);
     Timed tm(c, "gather", 0.0, bytes);
     launch_gather_items(di, dt, (int)tiles.size(), c->stream);
 }

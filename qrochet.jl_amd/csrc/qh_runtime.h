@@ -137,6 +137,17 @@ template <class T>
 T* upload(qh_context* c, const std::vector<T>& v) {
     return static_cast<T*>(arena_upload(c, v.data(), v.size() * sizeof(T)));
 }
+// several uploads with one host-to-device copy
+void arena_upload_segments(qh_context* c, int count, const void* const* srcs, const size_t* bytes, void** outs);
+template <class A, class B>
+void upload2(qh_context* c, const std::vector<A>& a, const std::vector<B>& b, A*& da, B*& db) {
+    const void* srcs[2] = {a.data(), b.data()};
+    const size_t bytes[2] = {a.size() * sizeof(A), b.size() * sizeof(B)};
+    void* outs[2];
+    arena_upload_segments(c, 2, srcs, bytes, outs);
+    da = static_cast<A*>(outs[0]);
+    db = static_cast<B*>(outs[1]);
+}
 // device -> host readback (synchronises)
 void readback(qh_context* c, void* host, const void* dev, size_t bytes);
 
