@@ -34,6 +34,19 @@ __device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
     }
 }
 
+// Loads through the global address space: the matrices live in device memory, and a
+// global load is tracked by the vector-memory counter alone, so waiting for LDS reads
+// does not also wait for loads still in flight (as it does for flat loads).
+typedef const __attribute__((address_space(1))) double* gconst_f64;
+__device__ __forceinline__ double2 gload(const double2* base, size_t idx) {
+    const gconst_f64 g = (gconst_f64)base;  // address-space cast (generic to global)
+    return make_double2(g[2 * idx], g[2 * idx + 1]);
+}
+// base[idx] where ok, else 0
+__device__ __forceinline__ double2 ld_or_zero(const double2* base, size_t idx, bool ok) {
+    return ok ? gload(base, idx) : make_double2(0.0, 0.0);
+}
+
 // Rotation of the Hermitian 2×2 problem [[g_aa, g_ab], [conj(g_ab), g_bb]]:
 // J = [[c, s e], [-s conj(e), c]] with e = g_ab/|g_ab| and t = s/c the smaller
 // root of t^2 + 2 zeta t - 1 = 0, zeta = (g_bb - g_aa)/(2|g_ab|) (Rutishauser's
@@ -162,7 +175,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const int e = tid + 256 * s2;
             const int c = e / CH, rr = e % CH;
             const int gc = sCol[c], grow = row0 + rr;
-            pre[s2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
+            pre[s2] = ld_or_zero(base, grow + (size_t)rows * gc, gc >= 0 && grow < rows);
         }
     };
     auto stash = [&]() {
@@ -452,7 +465,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int s2 = 0; s2 < UPT; ++s2) {
             const int e = tid + 256 * s2;
             const int gc = sCol[i0 + e / RB], grow = row0 + e % RB;
-            upre[s2] = (gc >= 0 && grow < rows) ? base[grow + (size_t)rows * gc] : cz();
+            upre[s2] = ld_or_zero(base, grow + (size_t)rows * gc, gc >= 0 && grow < rows);
         }
     };
     auto ustash = [&]() {
@@ -489,6 +502,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // A operands straight from HBM: the wave reads only its own rows (16
             // consecutive rows of one column per 16 lanes), so the update needs no
             // staging and no barrier; Q is read-only in LDS after the eigensolver.
+            // The loads are unconditional: a row past the end or an absent column reads
+            // an element in range instead -- absent columns meet zero rows of Q (their
+            // positions keep e_j), rows past the end are not stored -- so no branch
+            // holds a load back.
             const int r0w = row0 + 32 * mw + (ml & 15);
 #pragma unroll
             for (int st = 0; st < BC / 4; ++st) {
@@ -497,7 +514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     const int r = r0w + 16 * a;
-                    xa[a] = (c >= 0 && r < rows) ? base[r + (size_t)rows * c] : cz();
+                    xa[a] = gload(base, (size_t)min(r, rows - 1) + (size_t)rows * max(c, 0));
                 }
 #pragma unroll
                 for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
@@ -656,7 +673,7 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
                 const int e = tid + NT * s2;
                 const int c = e / CH, rr = e % CH;
                 const int gc = sCol[c], grow = row0 + rr;
-                pre[s2] = (gc >= 0 && grow < p) ? it.x[grow + (size_t)p * gc] : cz();
+                pre[s2] = ld_or_zero(it.x, grow + (size_t)p * gc, gc >= 0 && grow < p);
             }
         };
         fetch(0);
@@ -860,10 +877,19 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
 #pragma unroll
             for (int b = 0; b < 4; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
             const int rw = row0 + 16 * wv + (lane & 15);
-#pragma unroll 4
+            // all sixteen operand loads of the panel first, so they are in flight together
+            // (unconditional, in range: see the width-16 update)
+            double2 xall[BC / 4];
+#pragma unroll
             for (int st = 0; st < BC / 4; ++st) {
                 const int c = sCol[4 * st + (lane >> 4)];
-                const double2 xa = (c >= 0 && rw < rows) ? base[rw + (size_t)rows * c] : cz();
+                xall[st] = gload(base, (size_t)min(rw, rows - 1) + (size_t)rows * max(c, 0));
+            }
+#pragma unroll
+            for (int st = 0; st < BC / 4; ++st) {
+                // keep each step's Q reads in their step (the loads above stay in flight)
+                __builtin_amdgcn_sched_barrier(0);
+                const double2 xa = xall[st];
                 const double xs = xa.x + xa.y;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
