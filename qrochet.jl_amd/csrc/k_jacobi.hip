@@ -34,18 +34,6 @@ __device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
     }
 }
 
-// Loads through the global address space: the matrices live in device memory, and a
-// global load is tracked by the vector-memory counter alone, so waiting for LDS reads
-// does not also wait for loads still in flight (as it does for flat loads).
-typedef const __attribute__((address_space(1))) double* gconst_f64;
-__device__ __forceinline__ double2 gload(const double2* base, size_t idx) {
-    const gconst_f64 g = (gconst_f64)base;  // address-space cast (generic to global)
-    return make_double2(g[2 * idx], g[2 * idx + 1]);
-}
-// base[idx] where ok, else 0
-__device__ __forceinline__ double2 ld_or_zero(const double2* base, size_t idx, bool ok) {
-    return ok ? gload(base, idx) : make_double2(0.0, 0.0);
-}
 
 // Rotation of the Hermitian 2×2 problem [[g_aa, g_ab], [conj(g_ab), g_bb]]:
 // J = [[c, s e], [-s conj(e), c]] with e = g_ab/|g_ab| and t = s/c the smaller

@@ -329,7 +329,7 @@ __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelSha
     for (int j = 0; j < 2; ++j) {
         const int r = tid + NT * j;
 #pragma unroll
-        for (int k = 0; k < QR_NB; ++k) P[j][k] = (r < rows && k < nb) ? a0[r + lda * k] : cz();
+        for (int k = 0; k < QR_NB; ++k) P[j][k] = ld_or_zero(a0, r + lda * k, r < rows && k < nb);
     }
     double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
 #pragma unroll
@@ -407,7 +407,7 @@ __device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, in
     const double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
     for (int e = tid; e < QR_NB * QR_NB; e += 256) {
         const int i = e % QR_NB, j = e / QR_NB;
-        sT[i][j] = t[i + QR_NB * j];
+        sT[i][j] = gload(t, i + QR_NB * j);
     }
     // ---- phase 1: W = V^H Y
     const int lane = tid & 63, wv = tid >> 6;
@@ -426,8 +426,8 @@ __device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, in
             const int e = tid + 256 * s2;
             const int rr = e % RC, k = e / RC;
             const int r = r0 + rr;
-            pv[s2] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
-            py[s2] = (r < m && k < tc) ? it.y[r + (size_t)it.ldy * (c0 + k)] : cz();
+            pv[s2] = ld_or_zero(it.v, r + (size_t)m * (j0 + k), r < m && k < nb);
+            py[s2] = ld_or_zero(it.y, r + (size_t)it.ldy * (c0 + k), r < m && k < tc);
         }
     };
     fetch1(j0);
@@ -500,7 +500,7 @@ __device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, in
             const int e = tid + 256 * s2;
             const int rr = e % (2 * RC), k = e / (2 * RC);
             const int r = r0 + rr;
-            qv[s2] = (r < m && k < nb) ? it.v[r + (size_t)m * (j0 + k)] : cz();
+            qv[s2] = ld_or_zero(it.v, r + (size_t)m * (j0 + k), r < m && k < nb);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -508,7 +508,7 @@ __device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, in
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int c = 4 * cg + b;
-                qy[a][b] = (r < m && c < tc) ? it.y[r + (size_t)it.ldy * (c0 + c)] : cz();
+                qy[a][b] = ld_or_zero(it.y, r + (size_t)it.ldy * (c0 + c), r < m && c < tc);
             }
         }
     };
@@ -548,7 +548,7 @@ __device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, in
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int c = 4 * cg + b;
-                if (c < tc) it.y[r + (size_t)it.ldy * (c0 + c)] = csub(yc[a][b], o[a][b]);
+                if (c < tc) gstore(it.y, r + (size_t)it.ldy * (c0 + c), csub(yc[a][b], o[a][b]));
             }
         }
         __syncthreads();

@@ -30,4 +30,25 @@ __device__ __forceinline__ double2 cfmac(double2 a, double2 b, double2 c) {
 }
 __device__ __forceinline__ double cabs2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
 
+// Loads and stores through the global address space.  The matrices live in device
+// memory but reach the kernels as generic pointers inside descriptors, which compile
+// to flat accesses; a flat access is counted by the LDS counter as well, so every
+// wait for an LDS read also waits for the loads still in flight.  Global accesses
+// are counted by the vector-memory counter alone.
+typedef const __attribute__((address_space(1))) double* gconst_f64;
+typedef __attribute__((address_space(1))) double* gmut_f64;
+__device__ __forceinline__ double2 gload(const double2* base, size_t idx) {
+    const gconst_f64 g = (gconst_f64)base;  // address-space cast (generic to global)
+    return make_double2(g[2 * idx], g[2 * idx + 1]);
+}
+// base[idx] where ok, else 0
+__device__ __forceinline__ double2 ld_or_zero(const double2* base, size_t idx, bool ok) {
+    return ok ? gload(base, idx) : make_double2(0.0, 0.0);
+}
+__device__ __forceinline__ void gstore(double2* base, size_t idx, double2 v) {
+    const gmut_f64 g = (gmut_f64)base;
+    g[2 * idx] = v.x;
+    g[2 * idx + 1] = v.y;
+}
+
 }  // namespace qh
