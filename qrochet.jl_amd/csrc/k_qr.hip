@@ -17,75 +17,56 @@ namespace qh {
 
 constexpr int QR_TC = 16;  // target columns per apply workgroup
 
-// block-wide sum of K complex values per thread (NW waves); result in out[K]
 __device__ __forceinline__ double2 shfl_xor2(double2 a, int off) {
     return make_double2(__shfl_xor(a.x, off, 64), __shfl_xor(a.y, off, 64));
 }
 
-template <int K, int NW = 4>
-__device__ __forceinline__ void block_sum(double2 (&v)[K], double2* sRed /* NW*K */, double2* out /* K */) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if constexpr (K >= 2 && K <= 16 && (K & (K - 1)) == 0) {
-        // recursive halving: at the step with offset o the lane keeps the upper or
-        // lower half of its values (by lane bit o) and adds the partner's copy of
-        // that half, so K values need K-1 exchanges instead of K·6.  Afterwards lane
-        // l holds value k = (l >> s) & (K-1), s = 6 - log2 K, summed over the lanes
-        // that agree with l in bits s..5; the remaining bits are summed plainly.
-        int cnt = K;
-#pragma unroll
-        for (int off = 32; off > 32 / K; off >>= 1) {
-            const int half = cnt / 2;
-            const bool up = (lane & off) != 0;
-#pragma unroll
-            for (int i = 0; i < K / 2; ++i) {
-                if (i < half) {
-                    const double2 send = up ? v[i] : v[i + half];
-                    const double2 keep = up ? v[i + half] : v[i];
-                    v[i] = cadd(keep, shfl_xor2(send, off));
-                }
-            }
-            cnt = half;
-        }
-        double2 s = v[0];
-#pragma unroll
-        for (int off = 32 / K; off > 0; off >>= 1) s = cadd(s, shfl_xor2(s, off));
-        constexpr int SH = (K == 2) ? 5 : (K == 4) ? 4 : (K == 8) ? 3 : 2;
-        if ((lane & ((1 << SH) - 1)) == 0) sRed[wv * K + ((lane >> SH) & (K - 1))] = s;
-    } else {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double x = v[k].x, y = v[k].y;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                x += __shfl_xor(x, off, 64);
-                y += __shfl_xor(y, off, 64);
-            }
-            if (lane == 0) sRed[wv * K + k] = make_double2(x, y);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < K) {
-        double2 s = cz();
-        for (int w = 0; w < NW; ++w) s = cadd(s, sRed[w * K + threadIdx.x]);
-        out[threadIdx.x] = s;
-    }
-    __syncthreads();
+// the value of lane `src` (the same lane for the whole wave) in every lane
+__device__ __forceinline__ double bcast_d(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double2 bcast2(double2 v, int src) {
+    return make_double2(bcast_d(v.x, src), bcast_d(v.y, src));
 }
 
-// sum over the workgroup (NW waves) of one real value per thread, returned to every
-// thread (one barrier; `sRedW` must not be rewritten before the next barrier); the
-// wave sums are added as a pairwise tree
-template <int NW = 4>
-__device__ __forceinline__ double block_sum_all(double x, double* sRedW) {
-    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+// Sum over the workgroup (NW waves) of 16 complex values per thread, with one barrier.
+// Each wave halves its values recursively: at the step with offset o a lane keeps the
+// upper or the lower half (by lane bit o) and adds the partner's copy of that half, so
+// 16 values take 15 exchanges; lane l then holds value (l >> 2) & 15 summed over the
+// lanes that agree with l in bits 0-1, which two plain steps add.  The wave's 16 sums go
+// to sRed; after the barrier every lane adds the NW partials of value l & 15 in wave
+// order, so all waves hold the same totals, value k in lanes k, k + 16, k + 32, k + 48.
+// sRed is read after the barrier, so consecutive calls alternate two buffers: a fast
+// wave's next writes then cannot overtake a slow wave's reads.
+template <int NW>
+__device__ __forceinline__ double2 block_sum16(double2 (&v)[16], double2* sRed /* NW*16 */) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int cnt = 16;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    if (lane == 0) sRedW[wv] = x;
+    for (int off = 32; off > 2; off >>= 1) {
+        const int half = cnt / 2;
+        const bool up = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i < half) {
+                const double2 send = up ? v[i] : v[i + half];
+                const double2 keep = up ? v[i + half] : v[i];
+                v[i] = cadd(keep, shfl_xor2(send, off));
+            }
+        }
+        cnt = half;
+    }
+    double2 s = v[0];
+    s = cadd(s, shfl_xor2(s, 2));
+    s = cadd(s, shfl_xor2(s, 1));
+    if ((lane & 3) == 0) sRed[wv * 16 + ((lane >> 2) & 15)] = s;
     __syncthreads();
-    const double lo = (sRedW[0] + sRedW[1]) + (sRedW[2] + sRedW[3]);
-    if constexpr (NW == 4) return lo;
-    else return lo + ((sRedW[4] + sRedW[5]) + (sRedW[6] + sRedW[7]));
+    double2 t = sRed[lane & 15];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) t = cadd(t, sRed[w * 16 + (lane & 15)]);
+    return t;
 }
 
 // LAPACK zlarfg: H^H (alpha; x) = (beta; 0), H = I - tau v v^H, v = (1; x·scale), beta real
@@ -109,50 +90,43 @@ __device__ __forceinline__ Reflector make_reflector(double2 alpha, double xnorm2
     return h;
 }
 
+// The column step of a panel (column c, rows relative to the panel corner): one
+// workgroup sum gives, over the rows below the diagonal with x = column c there,
+// s_c = ||x||^2, s_k = x^H a_k (k > c) and s_k = v_k^H x (k < c); row c (alpha = a_c(c),
+// a_k(c), v_k(c)) is published by its owner before that sum's barrier.  With the
+// reflector v = (1; x·scale) the dots follow without a second sum:
+// v^H a_k = a_k(c) + conj(scale)·s_k and v_k^H v = conj(v_k(c)) + scale·s_k.
 struct PanelShared {
-    double2 sRed[8 * QR_NB];  // up to 8 waves
-    double sRedN[2][8];
-    double2 sDot[QR_NB];
+    double2 sRed[2][8 * QR_NB];  // wave partial sums (up to 8 waves), two alternating buffers
+    double2 sRow[2][QR_NB];      // row c of the panel, two alternating buffers
     double2 sT[QR_NB][QR_NB + 1];
-    double2 sScale;
-    double sBeta;
-    double2 sTau[QR_NB];
-    double2 sAlpha[2];
 };
+
+// the dot of slot k from the unscaled sum `tot` and row c's entry `row`: v^H a_k for
+// k > c, v_k^H v for k < c, 0 for k = c
+__device__ __forceinline__ double2 panel_dot(int k, int c, double2 tot, double2 row, double2 scale) {
+    if (k > c) return cadd(row, cmul(cconj(scale), tot));
+    if (k < c) return cadd(cconj(row), cmul(scale, tot));
+    return cz();
+}
 
 // Column loop of a panel factorisation on a panel view P(c, r) (column c, row r
 // relative to the panel corner) in global memory, NT threads.  Leaves R on and
 // above the diagonal, the scaled reflectors below it, and the compact-WY triangle
-// in sh.sT.
+// in sh.sT.  Thread tid owns rows r = tid + NT j for the whole panel, so the sums,
+// the reflector's scaling and the trailing update touch only the thread's own rows;
+// the only cross-thread data are the one sum per column and row c (PanelShared).
 template <int NT, class Ref>
 __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x;
-    // Thread tid owns rows r = tid + NT j for the whole panel, so the reflector's
-    // scaling, the dots, the trailing update and the next column's norm touch only
-    // the thread's own rows; the only cross-thread data are the reductions and
-    // alpha = P(c, c), whose owner writes it before the barrier of the next norm.
-    double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
-    for (int r = tid; r < rows; r += NT)
-        if (r > 0) part += cabs2(P(0, r));
     for (int c = 0; c < nb; ++c) {
-        // every thread forms the reflector from the reduced norm (no serial section);
-        // the two norm buffers alternate so a fast thread cannot overwrite partial
-        // sums that a slow thread of the previous column still reads
-        const double xnorm2 = block_sum_all<NW>(part, sh.sRedN[c & 1]);
-        const Reflector h = make_reflector(P(c, c), xnorm2);
-        // v below the diagonal (scaled in place) and its dots with the panel:
-        // dots[k] = v^H a_k (k > c), dots[k] = v_k^H v (k < c)
-        double2 dots[QR_NB];
+        double2 sk[QR_NB];
 #pragma unroll
-        for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
+        for (int k = 0; k < QR_NB; ++k) sk[k] = cz();
         for (int r = tid; r < rows; r += NT) {
-            if (r < c) continue;
-            double2 vr = make_double2(1.0, 0.0);
-            if (r > c) {
-                vr = cmul(P(c, r), h.scale);
-                P(c, r) = vr;
-            }
+            if (r <= c) continue;
+            const double2 x = P(c, r);
             // all 16 loads issue before their first use (column min(k, nb - 1) is always
             // inside the panel; the values of k >= nb are discarded)
             double2 pk[QR_NB];
@@ -160,20 +134,30 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
             for (int k = 0; k < QR_NB; ++k) pk[k] = P(min(k, nb - 1), r);
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) {
-                // k < c: v_k^H v (r >= c > k is below column k's diagonal); k > c: v^H a_k
-                const double2 t = (k < c) ? cfmac(pk[k], vr, dots[k]) : cfmac(vr, pk[k], dots[k]);
-                if (k != c && k < nb) dots[k] = t;
+                if (k < c) sk[k] = cfmac(pk[k], x, sk[k]);
+                else if (k == c) sk[k].x += cabs2(x);
+                else if (k < nb) sk[k] = cfmac(x, pk[k], sk[k]);
             }
         }
-        block_sum<QR_NB, NW>(dots, sh.sRed, sh.sDot);
-        // (every thread has read alpha = P(c, c) before the barriers above)
-        if (tid == c) P(c, c) = make_double2(h.beta, 0.0);
+        if (tid == c) {
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) sh.sRow[c & 1][k] = (k < nb && c < rows) ? P(k, c) : cz();
+        }
+        const double2 tot = block_sum16<NW>(sk, sh.sRed[c & 1]);
+        const Reflector h = make_reflector(sh.sRow[c & 1][c], bcast_d(tot.x, c));
+        const double2 dl = panel_dot(tid & 15, c, tot, sh.sRow[c & 1][tid & 15], h.scale);
+        double2 dk[QR_NB];
+#pragma unroll
+        for (int k = 0; k < QR_NB; ++k) dk[k] = bcast2(dl, k);
+        if (tid == c && c < rows) P(c, c) = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
             const int i = tid;
             double2 val = cz();
             if (i < c) {
                 double2 acc = cz();
-                for (int l = i; l < c; ++l) acc = cfma(sh.sT[i][l], sh.sDot[l], acc);
+#pragma unroll
+                for (int l = 0; l < QR_NB; ++l)
+                    if (l >= i && l < c) acc = cfma(sh.sT[i][l], dk[l], acc);
                 const double2 tc = h.tau;
                 val = make_double2(-(tc.x * acc.x - tc.y * acc.y), -(tc.x * acc.y + tc.y * acc.x));
             } else if (i == c) {
@@ -181,27 +165,25 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
             }
             sh.sT[i][c] = val;
         }
-        // trailing update a_k -= conj(tau) (v^H a_k) v on own rows, and the norm of
-        // the next column below its diagonal
+        // v below the diagonal (scaled in place) and the trailing update
+        // a_k -= conj(tau) (v^H a_k) v on own rows
         const double2 ctau = cconj(h.tau);
         double2 w[QR_NB];
 #pragma unroll
-        for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? cmul(ctau, sh.sDot[k]) : cz();
-        part = 0.0;
+        for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? cmul(ctau, dk[k]) : cz();
         for (int r = tid; r < rows; r += NT) {
             if (r < c) continue;
-            const double2 vr = (r == c) ? make_double2(1.0, 0.0) : P(c, r);
+            double2 vr = make_double2(1.0, 0.0);
+            if (r > c) {
+                vr = cmul(P(c, r), h.scale);
+                P(c, r) = vr;
+            }
             double2 pk[QR_NB];
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k) pk[k] = P(min(k, nb - 1), r);
 #pragma unroll
-            for (int k = 0; k < QR_NB; ++k) {
-                if (k > c && k < nb) {
-                    const double2 x = csub(pk[k], cmul(w[k], vr));
-                    P(k, r) = x;
-                    if (k == c + 1 && r > c + 1) part += cabs2(x);
-                }
-            }
+            for (int k = 0; k < QR_NB; ++k)
+                if (k > c && k < nb) P(k, r) = csub(pk[k], cmul(w[k], vr));
         }
     }
     __syncthreads();
@@ -240,25 +222,42 @@ __device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
 }
 
 // Panel factorisation with the panel in registers, for panels of up to 2·NT rows:
-// thread tid owns rows tid and tid + NT of the (rows × QR_NB) panel, so the dots and the trailing update of a
-// column read no memory.  The column index is a template parameter (one
-// instantiation per column), so every register index is static.  Cross-thread
-// data: the norm and dot reductions and alpha = P(c, c), which thread c (owner of
-// row c) publishes before the norm's barrier.  Same arithmetic, in the same
-// order, as qr_panel_columns.
+// thread tid owns rows tid and tid + NT of the (rows × QR_NB) panel, so the sums and
+// the trailing update of a column read no memory.  The column index is a template
+// parameter (one instantiation per column), so every register index is static.  Same
+// arithmetic, in the same order, as qr_panel_columns.
 template <int C, int NT>
-__device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& part, int rows, int nb,
-                                              PanelShared& sh) {
+__device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, int nb, PanelShared& sh) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x;
-    if (tid == C) sh.sAlpha[C & 1] = P[0][C];
-    const double xnorm2 = block_sum_all<NW>(part, sh.sRedN[C & 1]);
-    const Reflector h = make_reflector(sh.sAlpha[C & 1], xnorm2);
+    double2 sk[QR_NB];
+#pragma unroll
+    for (int k = 0; k < QR_NB; ++k) sk[k] = cz();
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int r = tid + NT * j;
+        if (r > C && r < rows) {
+            const double2 x = P[j][C];
+#pragma unroll
+            for (int k = 0; k < QR_NB; ++k) {
+                if (k < C) sk[k] = cfmac(P[j][k], x, sk[k]);
+                else if (k == C) sk[k].x += cabs2(x);
+                else if (k < nb) sk[k] = cfmac(x, P[j][k], sk[k]);
+            }
+        }
+    }
+    if (tid == C) {  // row C (< NT: the thread's first row)
+#pragma unroll
+        for (int k = 0; k < QR_NB; ++k) sh.sRow[C & 1][k] = P[0][k];
+    }
+    const double2 tot = block_sum16<NW>(sk, sh.sRed[C & 1]);
+    const Reflector h = make_reflector(sh.sRow[C & 1][C], bcast_d(tot.x, C));
+    const double2 dl = panel_dot(tid & 15, C, tot, sh.sRow[C & 1][tid & 15], h.scale);
+    double2 dk[QR_NB];
+#pragma unroll
+    for (int k = 0; k < QR_NB; ++k) dk[k] = bcast2(dl, k);
     // v on the owned rows (0 on rows above the diagonal and past the panel)
     double2 vr[2];
-    double2 dots[QR_NB];
-#pragma unroll
-    for (int k = 0; k < QR_NB; ++k) dots[k] = cz();
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int r = tid + NT * j;
@@ -268,15 +267,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
             vr[j] = cmul(P[j][C], h.scale);
             P[j][C] = vr[j];
         }
-        if (r >= C && r < rows) {
-#pragma unroll
-            for (int k = 0; k < QR_NB; ++k) {
-                if (k < C) dots[k] = cfmac(P[j][k], vr[j], dots[k]);
-                else if (k > C && k < nb) dots[k] = cfmac(vr[j], P[j][k], dots[k]);
-            }
-        }
     }
-    block_sum<QR_NB, NW>(dots, sh.sRed, sh.sDot);
     if (tid == C) P[0][C] = make_double2(h.beta, 0.0);
     if (tid < QR_NB) {
         const int i = tid;
@@ -285,7 +276,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
             double2 acc = cz();
 #pragma unroll
             for (int l = 0; l < C; ++l) {
-                const double2 t = cfma(sh.sT[i][l], sh.sDot[l], acc);
+                const double2 t = cfma(sh.sT[i][l], dk[l], acc);
                 if (l >= i) acc = t;
             }
             const double2 tc = h.tau;
@@ -295,25 +286,19 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], double& pa
         }
         sh.sT[i][C] = val;
     }
-    // trailing update a_k -= conj(tau) (v^H a_k) v, and the next column's norm
+    // trailing update a_k -= conj(tau) (v^H a_k) v
     const double2 ctau = cconj(h.tau);
-    part = 0.0;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int r = tid + NT * j;
         if (r >= C && r < rows) {
 #pragma unroll
-            for (int k = C + 1; k < QR_NB; ++k) {
-                if (k < nb) {
-                    const double2 x = csub(P[j][k], cmul(cmul(ctau, sh.sDot[k]), vr[j]));
-                    P[j][k] = x;
-                    if (k == C + 1 && r > C + 1) part += cabs2(x);
-                }
-            }
+            for (int k = C + 1; k < QR_NB; ++k)
+                if (k < nb) P[j][k] = csub(P[j][k], cmul(cmul(ctau, dk[k]), vr[j]));
         }
     }
     if constexpr (C + 1 < QR_NB) {
-        if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, part, rows, nb, sh);
+        if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, rows, nb, sh);
     }
 }
 
@@ -331,13 +316,7 @@ __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelSha
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) P[j][k] = ld_or_zero(a0, r + lda * k, r < rows && k < nb);
     }
-    double part = 0.0;  // ||column c below the diagonal||^2, this thread's rows
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int r = tid + NT * j;
-        if (r > 0 && r < rows) part += cabs2(P[j][0]);
-    }
-    qr_reg_column<0, NT>(P, part, rows, nb, sh);
+    qr_reg_column<0, NT>(P, rows, nb, sh);
     __syncthreads();
     // R (upper part) back to a, the explicit reflectors to v
 #pragma unroll

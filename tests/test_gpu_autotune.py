@@ -50,8 +50,9 @@ def test_explicit_settings_replace_the_tuned_schedule(qh):
     gates = [wl.haar_gate4(rng) for _ in range(n)]
     tuned, plain = qh.Context(), qh.Context()
     tuned.autotune(max_q=128, batch=4)
-    tuned.set_jacobi(block=16)
-    tuned.set_qr_passes(3)
+    for ctx in (tuned, plain):
+        ctx.set_jacobi(block=16)
+        ctx.set_qr_passes(3)
     out = []
     for ctx in (tuned, plain):
         psi = qh.from_internal(arrays, context=ctx)
