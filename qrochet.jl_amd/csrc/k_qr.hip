@@ -27,14 +27,55 @@ __device__ __forceinline__ double bcast_d(double x, int src) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
     return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double2 bcast2(double2 v, int src) {
-    return make_double2(bcast_d(v.x, src), bcast_d(v.y, src));
+
+// Cross-lane swaps of gfx950 (per dword of a double): permlane32 trades lanes 32..63
+// of a with lanes 0..31 of b, permlane16 trades the odd 16-lane rows of a (lanes
+// 16..31, 48..63) with the even rows of b (lanes 0..15, 32..47).  Afterwards a + b
+// holds, in every lane, its own value plus the partner's (lane ^ 32, resp. ^ 16) of the
+// half that the lane keeps: a's for the lower lanes, b's for the upper ones.
+__device__ __forceinline__ void swap_lanes32(double& a, double& b) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b),
+                                                     false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap_lanes16(double& a, double& b) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b),
+                                                     false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// halving step of block_sum16 with a lane swap: slot i (lower lanes) or i + half
+// (upper lanes), summed over the lane and its partner, lands in v[i]
+template <int OFF>
+__device__ __forceinline__ double2 halve_swap(double2 lo, double2 hi) {
+    if constexpr (OFF == 32) {
+        swap_lanes32(lo.x, hi.x);
+        swap_lanes32(lo.y, hi.y);
+    } else {
+        swap_lanes16(lo.x, hi.x);
+        swap_lanes16(lo.y, hi.y);
+    }
+    return cadd(lo, hi);
+}
+
+// LDS written by one wave and read back by the same wave: the fences order the
+// accesses across the wave's lanes, and no workgroup barrier is needed
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Sum over the workgroup (NW waves) of 16 complex values per thread, with one barrier.
 // Each wave halves its values recursively: at the step with offset o a lane keeps the
-// upper or the lower half (by lane bit o) and adds the partner's copy of that half, so
-// 16 values take 15 exchanges; lane l then holds value (l >> 2) & 15 summed over the
+// upper or the lower half (by lane bit o) and adds the partner's copy of that half (lane
+// swaps for o = 32, 16; shuffles for o = 8, 4), so 16 values take 15 exchanges; lane l
+// then holds value (l >> 2) & 15 summed over the
 // lanes that agree with l in bits 0-1, which two plain steps add.  The wave's 16 sums go
 // to sRed; after the barrier every lane adds the NW partials of value l & 15 in wave
 // order, so all waves hold the same totals, value k in lanes k, k + 16, k + 32, k + 48.
@@ -43,22 +84,28 @@ __device__ __forceinline__ double2 bcast2(double2 v, int src) {
 template <int NW>
 __device__ __forceinline__ double2 block_sum16(double2 (&v)[16], double2* sRed /* NW*16 */) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int cnt = 16;
+    double2 h8[8], h4[4];
 #pragma unroll
-    for (int off = 32; off > 2; off >>= 1) {
-        const int half = cnt / 2;
-        const bool up = (lane & off) != 0;
+    for (int i = 0; i < 8; ++i) h8[i] = halve_swap<32>(v[i], v[i + 8]);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (i < half) {
-                const double2 send = up ? v[i] : v[i + half];
-                const double2 keep = up ? v[i + half] : v[i];
-                v[i] = cadd(keep, shfl_xor2(send, off));
-            }
+    for (int i = 0; i < 4; ++i) h4[i] = halve_swap<16>(h8[i], h8[i + 4]);
+    double2 h2[2];
+    {
+        const bool up = (lane & 8) != 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double2 send = up ? h4[i] : h4[i + 2];
+            const double2 keep = up ? h4[i + 2] : h4[i];
+            h2[i] = cadd(keep, shfl_xor2(send, 8));
         }
-        cnt = half;
     }
-    double2 s = v[0];
+    double2 s;
+    {
+        const bool up = (lane & 4) != 0;
+        const double2 send = up ? h2[0] : h2[1];
+        const double2 keep = up ? h2[1] : h2[0];
+        s = cadd(keep, shfl_xor2(send, 4));
+    }
     s = cadd(s, shfl_xor2(s, 2));
     s = cadd(s, shfl_xor2(s, 1));
     if ((lane & 3) == 0) sRed[wv * 16 + ((lane >> 2) & 15)] = s;
@@ -99,8 +146,19 @@ __device__ __forceinline__ Reflector make_reflector(double2 alpha, double xnorm2
 struct PanelShared {
     double2 sRed[2][8 * QR_NB];  // wave partial sums (up to 8 waves), two alternating buffers
     double2 sRow[2][QR_NB];      // row c of the panel, two alternating buffers
+    double2 sBc[8][QR_NB];       // per wave: the 16 dots, written by lanes 0..15 and read by all
     double2 sT[QR_NB][QR_NB + 1];
 };
+
+// the dot of slot k (held by lane k of every wave) for every lane: lanes 0..15 write
+// the wave's own LDS row, which the wave then reads where it uses a dot (the next
+// column's writes follow these reads in the wave's program order, after a barrier)
+__device__ __forceinline__ const double2* panel_dots_publish(double2 dl, PanelShared& sh) {
+    double2* bc = sh.sBc[threadIdx.x >> 6];
+    if ((threadIdx.x & 63) < QR_NB) bc[threadIdx.x & 63] = dl;
+    wave_lds_sync();
+    return bc;
+}
 
 // the dot of slot k from the unscaled sum `tot` and row c's entry `row`: v^H a_k for
 // k > c, v_k^H v for k < c, 0 for k = c
@@ -146,9 +204,7 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
         const double2 tot = block_sum16<NW>(sk, sh.sRed[c & 1]);
         const Reflector h = make_reflector(sh.sRow[c & 1][c], bcast_d(tot.x, c));
         const double2 dl = panel_dot(tid & 15, c, tot, sh.sRow[c & 1][tid & 15], h.scale);
-        double2 dk[QR_NB];
-#pragma unroll
-        for (int k = 0; k < QR_NB; ++k) dk[k] = bcast2(dl, k);
+        const double2* dk = panel_dots_publish(dl, sh);
         if (tid == c && c < rows) P(c, c) = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
             const int i = tid;
@@ -233,17 +289,17 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     double2 sk[QR_NB];
 #pragma unroll
     for (int k = 0; k < QR_NB; ++k) sk[k] = cz();
+    // (x is 0 outside the rows below the diagonal, and the columns past the panel are 0,
+    // so every product is formed without a branch)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int r = tid + NT * j;
-        if (r > C && r < rows) {
-            const double2 x = P[j][C];
+        const double2 x = (r > C && r < rows) ? P[j][C] : cz();
 #pragma unroll
-            for (int k = 0; k < QR_NB; ++k) {
-                if (k < C) sk[k] = cfmac(P[j][k], x, sk[k]);
-                else if (k == C) sk[k].x += cabs2(x);
-                else if (k < nb) sk[k] = cfmac(x, P[j][k], sk[k]);
-            }
+        for (int k = 0; k < QR_NB; ++k) {
+            if (k < C) sk[k] = cfmac(P[j][k], x, sk[k]);
+            else if (k == C) sk[k].x += cabs2(x);
+            else sk[k] = cfmac(x, P[j][k], sk[k]);
         }
     }
     if (tid == C) {  // row C (< NT: the thread's first row)
@@ -253,9 +309,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     const double2 tot = block_sum16<NW>(sk, sh.sRed[C & 1]);
     const Reflector h = make_reflector(sh.sRow[C & 1][C], bcast_d(tot.x, C));
     const double2 dl = panel_dot(tid & 15, C, tot, sh.sRow[C & 1][tid & 15], h.scale);
-    double2 dk[QR_NB];
-#pragma unroll
-    for (int k = 0; k < QR_NB; ++k) dk[k] = bcast2(dl, k);
+    const double2* dk = panel_dots_publish(dl, sh);
     // v on the owned rows (0 on rows above the diagonal and past the panel)
     double2 vr[2];
 #pragma unroll
@@ -286,16 +340,14 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
         }
         sh.sT[i][C] = val;
     }
-    // trailing update a_k -= conj(tau) (v^H a_k) v
+    // trailing update a_k -= conj(tau) (v^H a_k) v (v is 0 off the reflector's rows and
+    // the dots of the columns past the panel are 0, so no branch is needed)
     const double2 ctau = cconj(h.tau);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int r = tid + NT * j;
-        if (r >= C && r < rows) {
+    for (int k = C + 1; k < QR_NB; ++k) {
+        const double2 wk = cmul(ctau, dk[k]);
 #pragma unroll
-            for (int k = C + 1; k < QR_NB; ++k)
-                if (k < nb) P[j][k] = csub(P[j][k], cmul(cmul(ctau, dk[k]), vr[j]));
-        }
+        for (int j = 0; j < 2; ++j) P[j][k] = csub(P[j][k], cmul(wk, vr[j]));
     }
     if constexpr (C + 1 < QR_NB) {
         if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, rows, nb, sh);
