@@ -416,180 +416,130 @@ __global__ __launch_bounds__(512) void k_qr_panel_tall(const QrItem* __restrict_
 }
 
 // Y[j0:, c0 : c0 + tc] <- (I - V T' V^H) Y[j0:, c0 : c0 + tc],  T' = T^H (adjoint) or T,
-// V and T of the panel of nb reflectors at j0.  W = V^H Y is accumulated in 4 × 4
-// register tiles (lane: row subgroup g, reflector group tk, column group tq) over
-// 64-row chunks staged in LDS, then summed over subgroups and waves; the update
-// Y -= V (T' W) gives each thread 2 rows × 4 columns of a 128-row chunk.
-struct ApplyShared {
-    static constexpr int RC = 64;  // rows per phase-1 chunk (phase 3: 2·RC)
-    double2 sBuf[2 * RC][QR_NB + 1];
+// V and T of the panel of nb reflectors at j0, on the matrix cores (v_mfma_f64_16x16x4,
+// three real products per complex product), operands straight from memory into registers.
+// Operand layouts of a 16x16x4 step: lane l gives A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register q of lane l is D[l / 16 + 4 q][l % 16].
+//   W = V^H Y: A[i][k] = conj V(r + k, j0 + i), B[k][j] = Y(r + k, c0 + j), four rows a
+//     step; wave wv takes steps wv, wv + 4, ..., loads of the next 8 steps in flight
+//     while 8 are multiplied.  The four waves' partial sums meet in LDS.
+//   W2 = T' W on the vector ALUs (one entry per thread).
+//   Y -= V W2 as (V W2)^T = W2^T V^T, so that a lane's results lie along one column
+//     of Y: A[j][k] = W2(k, j), B[k][i] = V(r0 + i, j0 + k); one 16-row block per
+//     step, wave wv takes blocks wv, wv + 4, ..., two at a time.
+struct ApplyMfmaShared {
+    double2 sPart[4][QR_NB][QR_TC];
     double2 sW[QR_NB][QR_TC + 1];
+    double2 sW2[QR_NB][QR_TC + 1];
     double2 sT[QR_NB][QR_NB + 1];
 };
-__device__ __forceinline__ void qr_apply_panel(const QrApplyItem& it, int c0, int tc, int j0, int nb, int adjoint,
-                                               ApplyShared& sh) {
+typedef double qr_f64x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c0, int tc, int j0, int nb,
+                                                    int adjoint, ApplyMfmaShared& sh) {
     static_assert(QR_TC == QR_NB && QR_NB == 16, "tiles are 16 reflectors × 16 columns");
-    constexpr int RC = ApplyShared::RC;
-    auto& sBuf = sh.sBuf;
-    auto& sW = sh.sW;
-    auto& sT = sh.sT;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int m = it.m;
+    const size_t ldy = it.ldy;
     const double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
-    for (int e = tid; e < QR_NB * QR_NB; e += 256) {
-        const int i = e % QR_NB, j = e / QR_NB;
-        sT[i][j] = gload(t, i + QR_NB * j);
-    }
-    // ---- phase 1: W = V^H Y
-    const int lane = tid & 63, wv = tid >> 6;
-    const int g = lane >> 4, tk = (lane >> 2) & 3, tq = lane & 3;
-    double2 acc[4][4];
+    for (int e = tid; e < QR_NB * QR_NB; e += 256) sh.sT[e % QR_NB][e / QR_NB] = gload(t, e);
+    // ---- phase 1: W = V^H Y over rows j0 .. m - 1
+    const int steps = (m - j0 + 3) / 4;
+    const int mine = (steps - wv + 3) / 4;  // this wave's steps: s = wv + 4 t, t < mine
+    const double2* vcol = it.v + (size_t)m * (j0 + li);
+    const double2* ycol = it.y + ldy * (c0 + li);
+    const bool vok = li < nb, yok = li < tc;
+    constexpr int U = 8;
+    double2 a0[U], b0[U], a1[U], b1[U];
+    auto fetch = [&](int t0, double2 (&a)[U], double2 (&b)[U]) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = cz();
-    // chunk r0 + RC is fetched into registers while chunk r0 is reduced from LDS
-    constexpr int E1 = RC * QR_NB / 256;
-    double2 pv[E1], py[E1];
-    auto fetch1 = [&](int r0) {
-#pragma unroll
-        for (int s2 = 0; s2 < E1; ++s2) {
-            const int e = tid + 256 * s2;
-            const int rr = e % RC, k = e / RC;
-            const int r = r0 + rr;
-            pv[s2] = ld_or_zero(it.v, r + (size_t)m * (j0 + k), r < m && k < nb);
-            py[s2] = ld_or_zero(it.y, r + (size_t)it.ldy * (c0 + k), r < m && k < tc);
+        for (int u = 0; u < U; ++u) {
+            const int r = j0 + 4 * (wv + 4 * (t0 + u)) + lk;
+            const bool ok = t0 + u < mine && r < m;
+            a[u] = ld_or_zero(vcol, r, ok && vok);
+            b[u] = ld_or_zero(ycol, r, ok && yok);
         }
     };
-    fetch1(j0);
-    for (int r0 = j0; r0 < m; r0 += RC) {
+    qr_f64x4 p = {0.0, 0.0, 0.0, 0.0}, q = p, s = p;
+    auto mult = [&](const double2 (&a)[U], const double2 (&b)[U]) {
 #pragma unroll
-        for (int s2 = 0; s2 < E1; ++s2) {
-            const int e = tid + 256 * s2;
-            sBuf[e % RC][e / RC] = pv[s2];
-            sBuf[RC + e % RC][e / RC] = py[s2];
+        for (int u = 0; u < U; ++u) {
+            p = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, b[u].x, p, 0, 0, 0);
+            q = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, b[u].y, q, 0, 0, 0);
+            s = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x - a[u].y, b[u].x + b[u].y, s, 0, 0, 0);
         }
-        __syncthreads();
-        if (r0 + RC < m) fetch1(r0 + RC);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int rr = 16 * wv + g + 4 * i;
-            double2 vv[4], yy[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                vv[a] = sBuf[rr][4 * tk + a];
-                yy[a] = sBuf[RC + rr][4 * tq + a];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = cfmac(vv[a], yy[b], acc[a][b]);
-        }
-        __syncthreads();
+    };
+    fetch(0, a0, b0);
+    for (int t0 = 0; t0 < mine; t0 += 2 * U) {
+        fetch(t0 + U, a1, b1);
+        mult(a0, b0);
+        if (t0 + 2 * U < mine) fetch(t0 + 2 * U, a0, b0);
+        mult(a1, b1);
     }
-    // sum over the four row subgroups of a wave, then over the waves (through sBuf)
-    double2* sPart = &sBuf[0][0];  // 4 × (QR_NB · QR_TC) values, fits in sBuf
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            acc[a][b] = cadd(acc[a][b], shfl_xor2(acc[a][b], 16));
-            acc[a][b] = cadd(acc[a][b], shfl_xor2(acc[a][b], 32));
-        }
-    if (g == 0) {
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) sPart[wv * QR_NB * QR_TC + (4 * tk + a) * QR_TC + 4 * tq + b] = acc[a][b];
-    }
+    for (int r = 0; r < 4; ++r)  // conj(a) b = (P + Q) + i (S - P + Q)
+        sh.sPart[wv][lk + 4 * r][li] = make_double2(p[r] + q[r], s[r] - p[r] + q[r]);
     __syncthreads();
     const int wa = tid / QR_TC, wc = tid % QR_TC;
-    {
-        const int e = wa * QR_TC + wc;
-        const double2 w = cadd(cadd(sPart[e], sPart[QR_NB * QR_TC + e]),
-                               cadd(sPart[2 * QR_NB * QR_TC + e], sPart[3 * QR_NB * QR_TC + e]));
-        sW[wa][wc] = w;
-    }
+    sh.sW[wa][wc] = cadd(cadd(sh.sPart[0][wa][wc], sh.sPart[1][wa][wc]),
+                         cadd(sh.sPart[2][wa][wc], sh.sPart[3][wa][wc]));
     __syncthreads();
     // ---- phase 2: W2 = T' W
     double2 w2 = cz();
     for (int b = 0; b < QR_NB; ++b) {
-        const double2 tab = adjoint ? cconj(sT[b][wa]) : sT[wa][b];
-        w2 = cfma(tab, sW[b][wc], w2);
+        const double2 tab = adjoint ? cconj(sh.sT[b][wa]) : sh.sT[wa][b];
+        w2 = cfma(tab, sh.sW[b][wc], w2);
     }
+    sh.sW2[wa][wc] = w2;
     __syncthreads();
-    sW[wa][wc] = w2;
-    __syncthreads();
-    // ---- phase 3: Y -= V W2   (thread: rows rt, rt + 64 of a chunk, columns 4·cg .. 4·cg + 3)
-    const int rt = tid & 63, cg = tid >> 6;
-    // V of chunk r0 + 2·RC and this thread's Y entries of it are fetched during chunk r0
-    constexpr int E3 = 2 * RC * QR_NB / 256;
-    double2 qv[E3], qy[2][4];
-    auto fetch3 = [&](int r0) {
+    // ---- phase 3: Y -= V W2, 16-row blocks
+    double2 wt[4];
 #pragma unroll
-        for (int s2 = 0; s2 < E3; ++s2) {
-            const int e = tid + 256 * s2;
-            const int rr = e % (2 * RC), k = e / (2 * RC);
-            const int r = r0 + rr;
-            qv[s2] = ld_or_zero(it.v, r + (size_t)m * (j0 + k), r < m && k < nb);
+    for (int k4 = 0; k4 < 4; ++k4) wt[k4] = sh.sW2[4 * k4 + lk][li];
+    const int nblk = (m - j0 + 15) / 16;
+    for (int blk0 = wv; blk0 < nblk; blk0 += 8) {
+        double2 vv[2][4], yo[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int blk = blk0 + 4 * h;
+            const int row = j0 + 16 * blk + li;
+            const bool rok = blk < nblk && row < m;
+#pragma unroll
+            for (int k4 = 0; k4 < 4; ++k4)
+                vv[h][k4] = ld_or_zero(it.v, row + (size_t)m * (j0 + 4 * k4 + lk), rok && 4 * k4 + lk < nb);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                yo[h][r] = ld_or_zero(it.y, row + ldy * (c0 + lk + 4 * r), rok && lk + 4 * r < tc);
         }
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int r = r0 + rt + RC * a;
+        for (int h = 0; h < 2; ++h) {
+            qr_f64x4 dp = {0.0, 0.0, 0.0, 0.0}, dq = dp, ds = dp;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int c = 4 * cg + b;
-                qy[a][b] = ld_or_zero(it.y, r + (size_t)it.ldy * (c0 + c), r < m && c < tc);
+            for (int k4 = 0; k4 < 4; ++k4) {
+                const double2 a = wt[k4], b = vv[h][k4];
+                dp = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, dp, 0, 0, 0);
+                dq = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, dq, 0, 0, 0);
+                ds = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, ds, 0, 0, 0);
+            }
+            const int blk = blk0 + 4 * h;
+            const int row = j0 + 16 * blk + li;
+            if (blk < nblk && row < m) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {  // a b = (P - Q) + i (S - P - Q)
+                    const int c = lk + 4 * r;
+                    if (c < tc)
+                        gstore(it.y, row + ldy * (c0 + c),
+                               csub(yo[h][r], make_double2(dp[r] - dq[r], ds[r] - dp[r] - dq[r])));
+                }
             }
         }
-    };
-    fetch3(j0);
-    for (int r0 = j0; r0 < m; r0 += 2 * RC) {
-#pragma unroll
-        for (int s2 = 0; s2 < E3; ++s2) {
-            const int e = tid + 256 * s2;
-            sBuf[e % (2 * RC)][e / (2 * RC)] = qv[s2];
-        }
-        double2 yc[2][4];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) yc[a][b] = qy[a][b];
-        __syncthreads();
-        if (r0 + 2 * RC < m) fetch3(r0 + 2 * RC);
-        double2 o[2][4];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) o[a][b] = cz();
-#pragma unroll 4
-        for (int k = 0; k < QR_NB; ++k) {
-            const double2 v0 = sBuf[rt][k], v1 = sBuf[rt + RC][k];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const double2 w = sW[k][4 * cg + b];
-                o[0][b] = cfma(v0, w, o[0][b]);
-                o[1][b] = cfma(v1, w, o[1][b]);
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int r = r0 + rt + RC * a;
-            if (r >= m) continue;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int c = 4 * cg + b;
-                if (c < tc) gstore(it.y, r + (size_t)it.ldy * (c0 + c), csub(yc[a][b], o[a][b]));
-            }
-        }
-        __syncthreads();
     }
 }
 
 // one block reflector (panel j0) on one column tile per workgroup
 __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
                                                   int j0, int adjoint) {
-    __shared__ ApplyShared sh;
+    __shared__ ApplyMfmaShared sh;
     const int2 tt = tiles[blockIdx.x];
     const QrApplyItem it = items[tt.x];
     const int nb = min(QR_NB, it.nrefl - j0);
@@ -597,20 +547,20 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
     const int c0 = (it.trailing ? j0 + nb : 0) + tt.y * QR_TC;
     const int tc = min(QR_TC, it.ncols - c0);
     if (tc <= 0) return;
-    qr_apply_panel(it, c0, tc, j0, nb, adjoint, sh);
+    qr_apply_panel_mfma(it, c0, tc, j0, nb, adjoint, sh);
 }
 
 // Y <- Q Y with every block reflector of the factorisation, last panel first: the
 // column tiles are independent, so one workgroup walks all panels of its tile.
 __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles) {
-    __shared__ ApplyShared sh;
+    __shared__ ApplyMfmaShared sh;
     const int2 tt = tiles[blockIdx.x];
     const QrApplyItem it = items[tt.x];
     const int c0 = tt.y * QR_TC;
     const int tc = min(QR_TC, it.ncols - c0);
     if (tc <= 0 || it.nrefl <= 0) return;
     for (int j0 = ((it.nrefl - 1) / QR_NB) * QR_NB; j0 >= 0; j0 -= QR_NB) {
-        qr_apply_panel(it, c0, tc, j0, min(QR_NB, it.nrefl - j0), 0, sh);
+        qr_apply_panel_mfma(it, c0, tc, j0, min(QR_NB, it.nrefl - j0), 0, sh);
         __syncthreads();
     }
 }
@@ -627,14 +577,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                  int j0) {
     __shared__ union {
         PanelShared p;
-        ApplyShared a;
+        ApplyMfmaShared a;
     } sh;
     if ((int)blockIdx.x < count) {
         const QrItem it = items[blockIdx.x];
         const int jn = j0 + QR_NB;
         if (jn >= it.n) return;
         const QrApplyItem ai = app[blockIdx.x];  // the same item's reflectors
-        qr_apply_panel(ai, jn, min(QR_TC, ai.ncols - jn), j0, QR_NB, 1, sh.a);
+        qr_apply_panel_mfma(ai, jn, min(QR_TC, ai.ncols - jn), j0, QR_NB, 1, sh.a);
         __syncthreads();
         qr_panel_regs<256>(it, jn, sh.p);  // at most 512 rows (host routing)
         return;
@@ -646,7 +596,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int c0 = j0 + nb + tt.y * QR_TC;
     const int tc = min(QR_TC, ai.ncols - c0);
     if (tc <= 0) return;
-    qr_apply_panel(ai, c0, tc, j0, nb, 1, sh.a);
+    qr_apply_panel_mfma(ai, c0, tc, j0, nb, 1, sh.a);
 }
 
 void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
