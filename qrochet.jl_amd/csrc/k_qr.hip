@@ -129,10 +129,11 @@ __device__ __forceinline__ Reflector make_reflector(double2 alpha, double xnorm2
     if (!(xnorm2 == 0.0 && alpha.y == 0.0)) {
         const double nrm = sqrt(cabs2(alpha) + xnorm2);
         h.beta = alpha.x >= 0.0 ? -nrm : nrm;
-        h.tau = make_double2((h.beta - alpha.x) / h.beta, -alpha.y / h.beta);
+        const double rb = 1.0 / h.beta;
+        h.tau = make_double2((h.beta - alpha.x) * rb, -alpha.y * rb);
         const double2 den = make_double2(alpha.x - h.beta, alpha.y);
-        const double dd = cabs2(den);
-        h.scale = make_double2(den.x / dd, -den.y / dd);
+        const double rd = 1.0 / cabs2(den);
+        h.scale = make_double2(den.x * rd, -den.y * rd);  // 1 / den
     }
     return h;
 }
@@ -150,7 +151,7 @@ struct PanelShared {
     double2 sT[QR_NB][QR_NB + 1];
 };
 
-// the dot of slot k (held by lane k of every wave) for every lane: lanes 0..15 write
+// the coefficient of slot k (held by lane k of every wave) for every lane: lanes 0..15 write
 // the wave's own LDS row, which the wave then reads where it uses a dot (the next
 // column's writes follow these reads in the wave's program order, after a barrier)
 __device__ __forceinline__ const double2* panel_dots_publish(double2 dl, PanelShared& sh) {
@@ -160,11 +161,12 @@ __device__ __forceinline__ const double2* panel_dots_publish(double2 dl, PanelSh
     return bc;
 }
 
-// the dot of slot k from the unscaled sum `tot` and row c's entry `row`: v^H a_k for
-// k > c, v_k^H v for k < c, 0 for k = c
-__device__ __forceinline__ double2 panel_dot(int k, int c, double2 tot, double2 row, double2 scale) {
-    if (k > c) return cadd(row, cmul(cconj(scale), tot));
-    if (k < c) return cadd(cconj(row), cmul(scale, tot));
+// slot k's coefficient from the unscaled sum `tot` and row c's entry `row`: for k > c
+// the trailing update's conj(tau) v^H a_k, for k < c the dot v_k^H v that builds T,
+// 0 for k = c
+__device__ __forceinline__ double2 panel_coef(int k, int c, double2 tot, double2 row, const Reflector& h) {
+    if (k > c) return cmul(cconj(h.tau), cadd(row, cmul(cconj(h.scale), tot)));
+    if (k < c) return cadd(cconj(row), cmul(h.scale, tot));
     return cz();
 }
 
@@ -203,7 +205,7 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
         }
         const double2 tot = block_sum16<NW>(sk, sh.sRed[c & 1]);
         const Reflector h = make_reflector(sh.sRow[c & 1][c], bcast_d(tot.x, c));
-        const double2 dl = panel_dot(tid & 15, c, tot, sh.sRow[c & 1][tid & 15], h.scale);
+        const double2 dl = panel_coef(tid & 15, c, tot, sh.sRow[c & 1][tid & 15], h);
         const double2* dk = panel_dots_publish(dl, sh);
         if (tid == c && c < rows) P(c, c) = make_double2(h.beta, 0.0);
         if (tid < QR_NB) {
@@ -223,10 +225,9 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
         }
         // v below the diagonal (scaled in place) and the trailing update
         // a_k -= conj(tau) (v^H a_k) v on own rows
-        const double2 ctau = cconj(h.tau);
         double2 w[QR_NB];
 #pragma unroll
-        for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? cmul(ctau, dk[k]) : cz();
+        for (int k = 0; k < QR_NB; ++k) w[k] = (k > c && k < nb) ? dk[k] : cz();
         for (int r = tid; r < rows; r += NT) {
             if (r < c) continue;
             double2 vr = make_double2(1.0, 0.0);
@@ -239,7 +240,7 @@ __device__ void qr_panel_columns(int rows, int nb, PanelShared& sh, Ref P) {
             for (int k = 0; k < QR_NB; ++k) pk[k] = P(min(k, nb - 1), r);
 #pragma unroll
             for (int k = 0; k < QR_NB; ++k)
-                if (k > c && k < nb) P(k, r) = csub(pk[k], cmul(w[k], vr));
+                if (k > c && k < nb) P(k, r) = cfms(w[k], vr, pk[k]);
         }
     }
     __syncthreads();
@@ -308,7 +309,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     }
     const double2 tot = block_sum16<NW>(sk, sh.sRed[C & 1]);
     const Reflector h = make_reflector(sh.sRow[C & 1][C], bcast_d(tot.x, C));
-    const double2 dl = panel_dot(tid & 15, C, tot, sh.sRow[C & 1][tid & 15], h.scale);
+    const double2 dl = panel_coef(tid & 15, C, tot, sh.sRow[C & 1][tid & 15], h);
     const double2* dk = panel_dots_publish(dl, sh);
     // v on the owned rows (0 on rows above the diagonal and past the panel)
     double2 vr[2];
@@ -342,12 +343,11 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     }
     // trailing update a_k -= conj(tau) (v^H a_k) v (v is 0 off the reflector's rows and
     // the dots of the columns past the panel are 0, so no branch is needed)
-    const double2 ctau = cconj(h.tau);
 #pragma unroll
     for (int k = C + 1; k < QR_NB; ++k) {
-        const double2 wk = cmul(ctau, dk[k]);
+        const double2 wk = dk[k];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) P[j][k] = csub(P[j][k], cmul(wk, vr[j]));
+        for (int j = 0; j < 2; ++j) P[j][k] = cfms(wk, vr[j], P[j][k]);
     }
     if constexpr (C + 1 < QR_NB) {
         if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, rows, nb, sh);

@@ -20,6 +20,14 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
     c.y = fma(a.y, b.x, c.y);
     return c;
 }
+// c - a * b
+__device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
+    c.x = fma(-a.x, b.x, c.x);
+    c.x = fma(a.y, b.y, c.x);
+    c.y = fma(-a.x, b.y, c.y);
+    c.y = fma(-a.y, b.x, c.y);
+    return c;
+}
 // c + conj(a) * b
 __device__ __forceinline__ double2 cfmac(double2 a, double2 b, double2 c) {
     c.x = fma(a.x, b.x, c.x);
