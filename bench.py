@@ -434,7 +434,7 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     ctx.set_timing(False)
-    classes = ["jacobi_round", "theta", "qr", "qr_apply", "select", "gather", "gemm", "jacobi_sweep"]
+    classes = ["jacobi_round", "jacobi_check", "theta", "qr", "qr_apply", "select", "gather", "gemm", "jacobi_sweep"]
     timers = {k: ctx.timing(k) for k in classes}
     elapsed = t1 - t0
     value = n_updates * args.steps / elapsed
@@ -463,7 +463,7 @@ def main():
             t = json.load(fh)
         roof["traffic"] = t["traffic_bytes_per_launch"]
         roof["traffic_source"] = f"profiles/k_jacobi_round_traffic.json ({t['workload']})"
-    roof["kernel"] = "k_jacobi_round<32> (block one-sided Jacobi round, FP64)"
+    roof["kernel"] = "k_jacobi_round<32, false> (block one-sided Jacobi round, FP64)"
     roof["avg_launch_ms"] = round(jr["ms"] / launches, 5)
     roof["launches"] = jr["launches"]
     roof["algorithmic_flops_per_launch"] = f_alg / launches

@@ -18,10 +18,18 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 namespace qh {
 
-__device__ __forceinline__ void rr_pair(int round, int k, int nidx, int& a, int& b) {
-    const int m = nidx - 1;
-    if (k == 0) { a = round; b = m; }
-    else { a = (round + k) % m; b = (round - k + m) % m; }
+// The tail criterion at the start of an item's own sweep s (after sweep w = s - 1): an
+// item of at least JAC_TAIL_NB blocks, past the sorting sweeps, not before
+// own[JAC_TAIL_MIN], whose sweep w rotated some but at most tail_switch per mille of its
+// block pairs (own[w mod 5]; a sweep without a rotation converges the item in the plain
+// loop).
+constexpr int JAC_TAIL_NB = 8;
+__device__ __forceinline__ bool tail_due(const JacItem& it, int s, int sort_sweeps, int tail_switch) {
+    const int w = s - 1;
+    if (tail_switch <= 0 || it.nb < JAC_TAIL_NB || w < sort_sweeps || s < it.own[JAC_TAIL_MIN]) return false;
+    const long long pairs = (long long)(it.nb - 1) * (it.nb / 2);
+    const long long cnt = it.own[w % 5];
+    return cnt > 0 && 1000LL * cnt <= tail_switch * pairs;
 }
 // rr_pair(round + 1, k, nidx) from rr_pair(round, k, nidx)
 __device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
@@ -92,12 +100,16 @@ __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j
     ya = na;
 }
 
-template <int BC>
+// CHECK: the test of every (round, pair) of a sweep in one launch, without rotating (its
+// own kernel, so that profiles and the round timings keep the rounds apart)
+template <int BC, bool CHECK>
 // three waves per SIMD: the MFMA accumulators fit in the VGPRs without spilling
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_jacobi_round(const JacItem* __restrict__ items,
                                                       const int2* __restrict__ tiles, int round_arg,
                                                       const int* __restrict__ round_base, int sort_sweeps,
-                                                      int debug, int max_inner) {
+                                                      int debug, int max_inner, int mode,
+                                                      int* __restrict__ flags, const int* __restrict__ flag0,
+                                                      int tail_switch) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
@@ -116,28 +128,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ int sSlot[BC], sOrd[BC];
     __shared__ double sKey[BC];
     __shared__ int sFlag;
+    __shared__ int sGo;
 
     const int tid = threadIdx.x;
     const int2 tt = tiles[blockIdx.x];
     const JacItem it = items[tt.x];
     if (*((volatile int*)it.conv)) return;
-    // the round of the tournament: an argument, or the sweep's first round kept in device
-    // memory plus the argument (so that a captured sweep replays with advancing rounds)
-    const int round = round_base ? *round_base + round_arg : round_arg;
     const int R = it.nb - 1;
-    const int r = round % R;
     const double tol = it.tol;
     const double tol2 = tol * tol;
     const double floor_rel2 = it.floor_rel2;
-    // eigenpair sorting during this matrix's own first sweeps (independent of the batch)
-    const int sort = round < sort_sweeps * R;
-    int bi, bj;
-    rr_pair(r, tt.y, it.nb, bi, bj);
+    int round, sort, bi, bj, own = 0;
+    if (tid == 0) sGo = 1;
+    if (mode == JAC_PAIRS) {
+        // an explicit block pair (tile: item, bi | bj << 16), no sorting
+        round = -1;
+        sort = 0;
+        bi = tt.y & 0xffff;
+        bj = tt.y >> 16;
+    } else if (CHECK) {
+        // every (round, pair) of a sweep in one launch (tile: item, round · nb/2 + pair)
+        const int np = it.nb / 2;
+        round = tt.y / np;
+        sort = 0;
+        rr_pair(round, tt.y % np, it.nb, bi, bj);
+    } else {
+        // the round of the tournament: an argument, or the sweep's first round kept in
+        // device memory plus the argument (so that a captured sweep replays with
+        // advancing rounds); eigenpair sorting during this matrix's own first sweeps
+        // (independent of the batch)
+        round = (round_base ? *round_base + round_arg : round_arg) - it.own[JAC_ROUND_OFF];
+        own = round / R;
+        sort = round < sort_sweeps * R;
+        rr_pair(round % R, tt.y, it.nb, bi, bj);
+        // One thread reads the item's tail state (which the item's workgroup 0 may set
+        // during this launch) and decides for the workgroup.  At the start of the item's
+        // own sweep `own`: the tail criterion on the block pairs its previous sweeps
+        // rotated (every workgroup reads the same counts and takes the same decision), and
+        // the count of the next sweep cleared.  Counted per own sweep, the decision does
+        // not depend on the rest of the batch.
+        if (tid == 0) {
+            const int ts = *((volatile int*)it.tail);
+            int go = ts != 1;
+            if (go && round % R == 0) {
+                if (ts == 0 && tail_due(it, own, sort_sweeps, tail_switch)) {
+                    go = 0;
+                    if (tt.y == 0) {
+                        *it.tail = 1;
+                        it.own[JAC_TAIL_AT] = own;
+                    }
+                }
+                if (go && tt.y == 0) it.own[(own + 1) % 5] = 0;
+            }
+            sGo = go;
+        }
+    }
     if (tid < BC) {
         const int c = (tid < B) ? bi * B + tid : bj * B + (tid - B);
         sCol[tid] = (c < it.q) ? c : -1;
     }
     __syncthreads();
+    if (!sGo) return;
 
     const int p = it.p;
     // columns at or below the rounding floor are frozen (treated as zero)
@@ -290,6 +341,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     }
     __syncthreads();
+    if constexpr (CHECK) {  // the test's outcome only
+        if (tid == 0) flags[flag0[tt.x] + tt.y] = sFlag;
+        return;
+    }
     if (!sFlag) return;
 
     // the eigensolver and the sort are a latency-bound chain; their waves get issue
@@ -374,7 +429,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int any = sFlag;
         __syncthreads();
         if (!any) break;
-        if (sweep == 0 && tid == 0) atomicAdd(it.rot, 1);  // counted only if a rotation happened
+        if (sweep == 0 && tid == 0) {  // counted only if a rotation happened
+            atomicAdd(it.rot, 1);
+            if (mode == JAC_ROUND) atomicAdd(&it.own[own % 5], 1);
+        }
     }
 
     // ---------------- order eigenpairs: larger eigenvalues to the lower block ----------------
@@ -903,14 +961,131 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
 }
 
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
-                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, hipStream_t s) {
+                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, int tail_switch,
+                         hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
-        k_jacobi_round<16><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
+        k_jacobi_round<16, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug,
+                                                         max_inner, JAC_ROUND, nullptr, nullptr,
+                                                         tail_switch);
     else if (block == 32)
         k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, max_inner);
     else
-        k_jacobi_round<32><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner);
+        k_jacobi_round<32, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug,
+                                                         max_inner, JAC_ROUND, nullptr, nullptr,
+                                                         tail_switch);
+}
+void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles, int* d_flags, const int* d_flag0,
+                         int block, hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (block == 8)
+        k_jacobi_round<16, true><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, 1, JAC_CHECK, d_flags, d_flag0,
+                                                        0);
+    else
+        k_jacobi_round<32, true><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, 1, JAC_CHECK, d_flags, d_flag0,
+                                                        0);
+}
+// Convergence test of whole items without rotating (block width 16): G = X^H X in
+// 64 × 64 tiles of the upper block triangle, one workgroup per tile, on the matrix cores
+// (v_mfma_f64_16x16x4, three real products per complex block, as in the rounds' Gram);
+// wave w holds the four 16 × 16 sub-tiles of block row w of the tile, i.e. the block
+// pairs (I, J) = (4·ti + w, 4·tj + b).  The column norms come from the same LDS chunks.
+// flags[first[item] + I·nb + J] = 1 where a column pair of blocks I <= J fails the
+// rotation test (for I = J the pairs inside the block), 0 otherwise; the blocks past the
+// item and the lower half of a diagonal tile are not written.  Rows are streamed in
+// chunks of 16, the next chunk fetched while the current one is multiplied.
+constexpr int JCT = 64, JCR = 16;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jacobi_check_tiles(const JacItem* __restrict__ items,
+                                                            const int4* __restrict__ tiles, int* __restrict__ flags,
+                                                            const int* __restrict__ first) {
+    __shared__ double2 sX[2 * JCT][JCR + 1];
+    __shared__ double sN[2 * JCT];
+    const int4 tt = tiles[blockIdx.x];
+    const JacItem it = items[tt.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int p = it.p, q = it.q;
+    const int c0 = tt.y * JCT, c1 = tt.z * JCT;
+    // thread tid stages rows e % JCR of column e / JCR, e = tid + 256 s (side I for the
+    // columns below JCT, side J above)
+    constexpr int LPT = 2 * JCT * JCR / 256;
+    double2 pre[LPT];
+    auto fetch = [&](int row0) {
+#pragma unroll
+        for (int s2 = 0; s2 < LPT; ++s2) {
+            const int e = tid + 256 * s2;
+            const int cl = e / JCR, r = row0 + e % JCR;
+            const int col = (cl < JCT) ? c0 + cl : c1 + cl - JCT;
+            pre[s2] = ld_or_zero(it.x, (size_t)min(r, p - 1) + (size_t)p * min(col, q - 1), r < p && col < q);
+        }
+    };
+    f64x4 P[4], Q[4], S[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) P[b] = Q[b] = S[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double nrm = 0.0;  // thread tid < 2·JCT: |x|^2 of column tid of the tile pair
+    fetch(0);
+    for (int row0 = 0; row0 < p; row0 += JCR) {
+#pragma unroll
+        for (int s2 = 0; s2 < LPT; ++s2) {
+            const int e = tid + 256 * s2;
+            sX[e / JCR][e % JCR] = pre[s2];
+        }
+        __syncthreads();
+        if (row0 + JCR < p) fetch(row0 + JCR);
+        if (tid < 2 * JCT) {
+#pragma unroll
+            for (int r = 0; r < JCR; ++r) nrm += cabs2(sX[tid][r]);
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < JCR / 4; ++s4) {
+            const int r = 4 * s4 + (lane >> 4);
+            const double2 xa = sX[16 * wv + (lane & 15)][r];
+            const double da = xa.x - xa.y;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const double2 xb = sX[JCT + 16 * b + (lane & 15)][r];
+                P[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, xb.x, P[b], 0, 0, 0);
+                Q[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, xb.y, Q[b], 0, 0, 0);
+                S[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(da, xb.x + xb.y, S[b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 2 * JCT) sN[tid] = nrm;
+    __syncthreads();
+    const double tol2 = it.tol * it.tol;
+    const double gfloor = it.floor_rel2 * (*it.fro2);
+    const bool diag = tt.y == tt.z;
+    const int I = 4 * tt.y + wv;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int J = 4 * tt.z + b;
+        if (I >= it.nb || J >= it.nb || (diag && b < wv)) continue;  // wave-uniform
+        bool bad = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = (lane >> 4) + 4 * r, j = lane & 15;  // result register r of the lane
+            if (diag && b == wv && i >= j) continue;
+            const double2 g = make_double2(P[b][r] + Q[b][r], S[b][r] - P[b][r] + Q[b][r]);
+            bad = bad || needs_rotation(sN[16 * wv + i], sN[JCT + 16 * b + j], g, tol2, gfloor);
+        }
+        const bool any = __ballot(bad) != 0;
+        if (lane == 0) flags[first[tt.x] + I * it.nb + J] = any ? 1 : 0;
+    }
+}
+void launch_jacobi_check_tiles(const JacItem* d_items, const int4* d_tiles, int ntiles, int* d_flags,
+                               const int* d_first, hipStream_t s) {
+    if (ntiles <= 0) return;
+    k_jacobi_check_tiles<<<ntiles, 256, 0, s>>>(d_items, d_tiles, d_flags, d_first);
+}
+void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles, int block, int max_inner,
+                         hipStream_t s) {
+    if (ntiles <= 0) return;
+    if (block == 8)
+        k_jacobi_round<16, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, max_inner, JAC_PAIRS,
+                                                         nullptr, nullptr, 0);
+    else
+        k_jacobi_round<32, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, max_inner, JAC_PAIRS,
+                                                         nullptr, nullptr, 0);
 }
 
 // ||X||_F^2 of every item, deterministic: each workgroup reduces one chunk into
@@ -957,7 +1132,12 @@ __global__ void k_set_identity(const JacItem* __restrict__ items, const int2* __
     if (col >= it.q) return;
     for (int r = threadIdx.x; r < it.q; r += blockDim.x)
         it.v[r + (size_t)it.q * col] = make_double2(r == col ? 1.0 : 0.0, 0.0);
-    if (threadIdx.x == 0) { *it.rot = 0; *it.conv = 0; }
+    if (threadIdx.x == 0) {
+        *it.rot = 0;
+        *it.conv = 0;
+        *it.tail = 0;
+        for (int k = 0; k < JAC_OWN_INTS; ++k) it.own[k] = 0;
+    }
 }
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s) {
     if (ntiles <= 0) return;
@@ -968,15 +1148,23 @@ void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int 
 // receives the number of items still rotating, remaining[1] the number with
 // non-finite entries.
 __global__ void k_conv_update(const JacItem* __restrict__ items, int count, int* remaining, int* round_base,
-                              int rounds) {
-    __shared__ int sRem, sBad;
-    if (threadIdx.x == 0) sRem = sBad = 0;
+                              int rounds, int first_round, int sort_sweeps, int tail_switch) {
+    __shared__ int sRem, sBad, sTail;
+    if (threadIdx.x == 0) sRem = sBad = sTail = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         const JacItem it = items[i];
         const int cv = *it.conv;
         if (cv == 0) {
-            if (*it.rot == 0) *it.conv = 1;
+            // an item whose own sweep ends with the window takes the round kernel's tail
+            // decision now, so that the next window does not start with it
+            const int R = it.nb - 1, end = first_round + rounds - it.own[JAC_ROUND_OFF];
+            if (*it.tail == 0 && *it.rot > 0 && end % R == 0 && tail_due(it, end / R, sort_sweeps, tail_switch)) {
+                *it.tail = 1;
+                it.own[JAC_TAIL_AT] = end / R;
+            }
+            if (*it.tail == 1) atomicAdd(&sTail, 1);  // waits for the tail passes
+            else if (*it.rot == 0) *it.conv = 1;
             else atomicAdd(&sRem, 1);
         } else if (cv < 0) {
             atomicAdd(&sBad, 1);
@@ -987,12 +1175,14 @@ __global__ void k_conv_update(const JacItem* __restrict__ items, int count, int*
     if (threadIdx.x == 0) {
         remaining[0] = sRem;
         remaining[1] = sBad;
+        remaining[3] = sTail;
         if (round_base) *round_base += rounds;  // the next sweep's first round
     }
 }
 void launch_conv_update(const JacItem* d_items, int count, int* remaining, int* round_base, int rounds,
-                        hipStream_t s) {
-    k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining, round_base, rounds);
+                        int first_round, int sort_sweeps, int tail_switch, hipStream_t s) {
+    k_conv_update<<<1, 256, 0, s>>>(d_items, count, remaining, round_base, rounds, first_round, sort_sweeps,
+                                    tail_switch);
 }
 
 // ---------------------------------------------------------------------------

@@ -1100,6 +1100,7 @@ int qh_ctx_create(int device, qh_context** out) {
         QH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         if (const char* e = std::getenv("QH_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         if (const char* e = std::getenv("QH_PIVOT")) c->pivot = std::atoi(e) != 0;
+        if (const char* e = std::getenv("QH_JACOBI_TAIL")) c->jac_tail = std::atoi(e) != 0;
         if (const char* e = std::getenv("QH_JACOBI_BLOCK")) {
             const int b = std::atoi(e);
             if (b == 8 || b == 16 || b == 32) c->jac_block = b;

@@ -102,10 +102,18 @@ struct JacItem {
     int nb;         // column blocks (even >= 2)
     int* rot;       // rotations performed in the current window (device counter)
     int* conv;      // 1 once converged (kernels skip the item)
+    int* tail;      // 1 while the item is in the tail passes (the rounds skip it), 2 once it left them
+    int* own;       // JAC_OWN_INTS ints: rotated block pairs of the item's own sweeps (by
+                    // sweep index mod 5), and the tail's bookkeeping (below)
     double* fro2;   // ||X||_F^2 (device scalar, filled before the sweeps)
     double tol;     // relative orthogonality target of this matrix
     double floor_rel2;  // deflation floor relative to ||X||_F^2
 };
+// JacItem::own: [0, 5) rotated block pairs by own sweep mod 5; the item's round offset
+// (its own round = the batch's round - offset, set when it returns from the tail so that
+// it resumes its own sweep at its first round); the own sweep at which it entered the
+// tail; the first own sweep at which it may enter the tail.
+constexpr int JAC_ROUND_OFF = 5, JAC_TAIL_AT = 6, JAC_TAIL_MIN = 7, JAC_OWN_INTS = 8;
 // Columns with ||x||^2 <= JAC_FLOOR(p)^2 ||X||_F^2 are below the rounding floor of the
 // rotations and are treated as zero (frozen): JAC_FLOOR(p) = 4 sqrt(p) eps.
 // ||X||_F^2 per item (tile map: item, chunk of 64k elements)
@@ -114,14 +122,46 @@ void launch_frob2(const JacItem* d_items, int count, const int2* d_tiles, int nt
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 // round: the tournament round, plus *round_base when round_base is not null
+// tail_switch > 0 (per mille, block 8 and 16): at the start of each of its own sweeps after
+// the sorting ones, an item whose previous sweep rotated at most that share of its block
+// pairs stops (tail = 1) and waits for the tail passes (JAC_CHECK / JAC_PAIRS launches)
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
-                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, hipStream_t s);
+                         const int* round_base, int block, int sort_sweeps, int debug, int max_inner, int tail_switch,
+                         hipStream_t s);
+// Pair k of round `round` of the circle-method tournament over nidx (even) blocks.
+__host__ __device__ inline void rr_pair(int round, int k, int nidx, int& a, int& b) {
+    const int m = nidx - 1;
+    if (k == 0) {
+        a = round;
+        b = m;
+    } else {
+        a = (round + k) % m;
+        b = (round - k + m) % m;
+    }
+}
+// Modes of the block-8/16 round kernel: a tournament round; the test of every (round,
+// pair) of a sweep in one launch, without rotating (tile: item, round · nb/2 + pair;
+// flags[flag0[item] + tile.y] = 1 where the pair needs a rotation); explicit block
+// pairs (tile: item, bi | bj << 16), which must be disjoint within an item.
+enum JacMode { JAC_ROUND = 0, JAC_CHECK = 1, JAC_PAIRS = 2 };
+void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles, int* d_flags, const int* d_flag0,
+                         int block, hipStream_t s);
+void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles, int block, int max_inner,
+                         hipStream_t s);
+// block width 16: the test of every column pair of whole items (tile: item, ti, tj with
+// ti <= tj, 64-column tiles of the upper block triangle); flags[first[item] + I·nb + J]
+// for the blocks I <= J of the tile
+void launch_jacobi_check_tiles(const JacItem* d_items, const int4* d_tiles, int ntiles, int* d_flags,
+                               const int* d_first, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
-// converged <- (no rotation in the window); *remaining = items still rotating
-// and, if round_base is not null, *round_base += rounds (the next sweep's first round)
+// converged <- (no rotation in the window, outside the tail); remaining[0] = items still
+// rotating, remaining[1] = items with non-finite entries, remaining[3] = items waiting
+// for the tail passes (an item whose own sweep ends with the window, at first_round +
+// rounds, takes the round kernel's tail decision here); if round_base is not null,
+// *round_base += rounds (the next sweep's first round)
 void launch_conv_update(const JacItem* d_items, int count, int* remaining, int* round_base, int rounds,
-                        hipStream_t s);
+                        int first_round, int sort_sweeps, int tail_switch, hipStream_t s);
 
 // Finalisation: σ_j = ||X[:, j]||, sort descending, truncation rule of
 // `truncate!`, optional renormalisation, kept count and discarded weight.

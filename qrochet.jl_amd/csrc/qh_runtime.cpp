@@ -6,6 +6,7 @@
 #include <array>
 #include <cfloat>
 #include <cmath>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -424,8 +425,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     const int bc = 2 * block;
     std::vector<int2> id_tiles, tiles;
     int rmax = 1;
-    int* counters = dalloc<int>(c, 2 * items.size() + 3);
-    QH_HIP(hipMemsetAsync(counters, 0, (2 * items.size() + 3) * sizeof(int), c->stream));
+    // per item: rotations in the window, converged, tail state, JacItem::own
+    constexpr int CS = 3 + JAC_OWN_INTS;
+    int* counters = dalloc<int>(c, CS * items.size() + 4);
+    QH_HIP(hipMemsetAsync(counters, 0, (CS * items.size() + 4) * sizeof(int), c->stream));
     double* fro2 = fro2_out ? fro2_out : dalloc<double>(c, items.size());
     auto free_fro2 = [&] {
         if (!fro2_out) dfree(c, fro2);
@@ -438,8 +441,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         if (nb < 2) nb = 2;
         if (nb & 1) ++nb;
         it.nb = nb;
-        it.rot = counters + 2 * i;
-        it.conv = counters + 2 * i + 1;
+        it.rot = counters + CS * i;
+        it.conv = counters + CS * i + 1;
+        it.tail = counters + CS * i + 2;
+        it.own = counters + CS * i + 3;
         it.fro2 = fro2 + i;
         const size_t tot = (size_t)it.p * it.q;
         for (size_t k = 0; k * FROB_CHUNK < tot; ++k) frob_tiles.push_back(make_int2((int)i, (int)k));
@@ -448,7 +453,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         for (int k = 0; k < it.q; ++k) id_tiles.push_back(make_int2((int)i, k));
         for (int k = 0; k < nb / 2; ++k) tiles.push_back(make_int2((int)i, k));
     }
-    int* remaining = counters + 2 * items.size();
+    int* remaining = counters + CS * items.size();
     // Relative orthogonality target |x_i^H x_j| <= tol ||x_i|| ||x_j||.  The Gram
     // entries carry O(sqrt(p)·ε) rounding noise, so the target sits a factor 16
     // above it; otherwise noise-triggered rotations never stop.  Columns below
@@ -477,8 +482,177 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         upd_b[i] = 32.0 * bc * (p + q);
     }
     std::vector<char> converged(items.size(), 0);
-    std::vector<int> hrot(2 * items.size());
+    std::vector<int> hrot(CS * items.size());
     bool done = false;
+    // The tail.  Once one of its own sweeps (after the sorting ones) rotated at most
+    // TAIL_SWITCH of its block pairs -- near the quadratic phase, where the next sweep
+    // rotates only a few -- an item leaves the rounds (the round kernel decides, per item,
+    // so the decision does not depend on the batch), and after the window its remaining
+    // sweeps become passes of: one launch testing every (round, pair) of a sweep without
+    // rotating; its flagged pairs grouped on the host into sets of disjoint pairs
+    // (greedily, in round order); one launch per set, where each pair is tested again on
+    // its fresh Gram before it rotates.  A pass whose test flags nothing is a full sweep
+    // without a rotation: the item has converged, as in the plain loop.  An item whose
+    // first test flags more than TAIL_MAX of its pairs returns to plain sweeps where it
+    // left them (its round offset makes its next round the first of the sweep it had not
+    // run; it may enter the tail again one sweep later), and so does an item still
+    // flagged after TAIL_PASSES passes (for good).
+    constexpr int TAIL_SWITCH = 700, TAIL_MAX = 250;  // per mille of an item's block pairs
+    constexpr int TAIL_PASSES = 8;
+    static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
+    const int tail_switch = (c->jac_tail && block != 32 && !kdbg) ? TAIL_SWITCH : 0;
+    int* tail_flags = nullptr;
+    std::vector<int> tail_first(items.size());
+    size_t tail_total = 0;
+    // the test: by 64-column tiles of X^H X at block width 16 (flags per block pair I, J of
+    // an nb × nb table), per (round, pair) of a sweep otherwise
+    const bool tile_check = block == 16;
+    for (size_t i = 0; i < items.size(); ++i) {
+        tail_first[i] = (int)tail_total;
+        const size_t nb = items[i].nb;
+        tail_total += tile_check ? nb * nb : (nb - 1) * (nb / 2);
+    }
+    // the items waiting for the tail (tail = 1) run their passes; next_round is the batch's
+    // first round of the next window
+    auto run_tail = [&](int next_round) {
+        if (!tail_flags) tail_flags = dalloc<int>(c, tail_total);
+        const int* d_first = upload(c, tail_first);
+        std::vector<int> hcnt(CS * items.size()), hflags(tail_total);
+        std::vector<char> had_sets(items.size(), 0);
+        auto publish = [&] {  // converged items, tail states, offsets, cleared rotation counts
+            QH_HIP(hipMemcpyAsync(counters, hcnt.data(), hcnt.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  c->stream));
+            QH_HIP(hipStreamSynchronize(c->stream));  // hcnt is rewritten by the next pass
+        };
+        // back to plain sweeps at the sweep it had not run; `again`: the first own sweep at
+        // which it may enter the tail once more
+        auto leave = [&](size_t i, int again) {
+            int* o = &hcnt[CS * i + 3];
+            hcnt[CS * i + 2] = 0;
+            o[JAC_ROUND_OFF] = next_round - o[JAC_TAIL_AT] * (items[i].nb - 1);
+            o[JAC_TAIL_MIN] = again;
+        };
+        for (int pass = 0;; ++pass) {
+            readback(c, hcnt.data(), counters, hcnt.size() * sizeof(int));
+            std::vector<int2> ctiles;
+            std::vector<int4> ttiles;
+            for (size_t i = 0; i < items.size(); ++i) {
+                if (hcnt[CS * i + 1] != 0 || hcnt[CS * i + 2] != 1) continue;
+                if (c->timing) {  // the rotations of the previous pass's sets
+                    c->timers["jacobi_round"].flops += hcnt[CS * i] * upd_f[i];
+                    c->timers["jacobi_round"].bytes += hcnt[CS * i] * upd_b[i];
+                }
+                if (had_sets[i] && hcnt[CS * i] == 0) {
+                    // every flagged pair passed the rounds' own test on its fresh Gram and
+                    // nothing moved: the two tests differ only in rounding at the margin
+                    hcnt[CS * i + 1] = 1;
+                    continue;
+                }
+                const int nb = items[i].nb;
+                if (tile_check) {
+                    const int nt = (nb + 3) / 4;
+                    for (int ti = 0; ti < nt; ++ti)
+                        for (int tj = ti; tj < nt; ++tj) ttiles.push_back(make_int4((int)i, ti, tj, 0));
+                } else {
+                    for (int e = 0; e < (nb - 1) * (nb / 2); ++e) ctiles.push_back(make_int2((int)i, e));
+                }
+            }
+            if (ctiles.empty() && ttiles.empty()) {
+                publish();
+                return;
+            }
+            if (pass == TAIL_PASSES) {
+                for (size_t i = 0; i < items.size(); ++i) {
+                    hcnt[CS * i] = 0;
+                    if (hcnt[CS * i + 1] == 0 && hcnt[CS * i + 2] == 1) leave(i, INT_MAX);
+                }
+                publish();
+                return;
+            }
+            {
+                Timed tm(c, "jacobi_check");
+                if (c->timing) c->timers["jacobi_check"].launches += 1;
+                if (tile_check)
+                    launch_jacobi_check_tiles(di, upload(c, ttiles), (int)ttiles.size(), tail_flags, d_first,
+                                              c->stream);
+                else
+                    launch_jacobi_check(di, upload(c, ctiles), (int)ctiles.size(), tail_flags, d_first, block,
+                                        c->stream);
+            }
+            readback(c, hflags.data(), tail_flags, tail_total * sizeof(int));
+            // per set: the tiles of the flagged pairs; per item and set: the blocks taken
+            std::vector<std::vector<int2>> sets;
+            for (size_t i = 0; i < items.size(); ++i) {
+                hcnt[CS * i] = 0;
+                if (hcnt[CS * i + 1] != 0 || hcnt[CS * i + 2] != 1) continue;
+                const int nb = items[i].nb, np = nb / 2, cnt = (nb - 1) * np;
+                if (c->timing) {
+                    c->timers["jacobi_check"].flops += cnt * gram_f[i];
+                    c->timers["jacobi_check"].bytes += cnt * gram_b[i];
+                }
+                // the flagged block pairs, in round order; at width 16 a block with a pair
+                // inside it failing and in no flagged pair rotates with its neighbour I ^ 1
+                std::vector<int2> fl;
+                const int* hf = &hflags[tail_first[i]];
+                if (tile_check) {
+                    std::vector<char> in(nb, 0), want(nb, 0);
+                    for (int I = 0; I < nb; ++I) {
+                        want[I] = hf[I * nb + I] != 0;
+                        for (int J = I + 1; J < nb; ++J)
+                            if (hf[I * nb + J]) in[I] = in[J] = 1;
+                    }
+                    for (int e = 0; e < cnt; ++e) {
+                        int a, b;
+                        rr_pair(e / np, e % np, nb, a, b);
+                        const int lo = std::min(a, b), hi = std::max(a, b);
+                        if (hf[lo * nb + hi] || (hi == (lo ^ 1) && ((want[lo] && !in[lo]) || (want[hi] && !in[hi]))))
+                            fl.push_back(make_int2(a, b));
+                    }
+                } else {
+                    for (int e = 0; e < cnt; ++e) {
+                        if (!hf[e]) continue;
+                        int a, b;
+                        rr_pair(e / np, e % np, nb, a, b);
+                        fl.push_back(make_int2(a, b));
+                    }
+                }
+                const int flagged = (int)fl.size();
+                if (flagged == 0) {
+                    hcnt[CS * i + 1] = 1;  // a full sweep without a rotation
+                    continue;
+                }
+                if (pass == 0 && 1000LL * flagged > (long long)TAIL_MAX * cnt) {
+                    leave(i, hcnt[CS * i + 3 + JAC_TAIL_AT] + 1);
+                    continue;
+                }
+                had_sets[i] = 1;
+                std::vector<std::vector<char>> taken;
+                for (const int2 pr : fl) {
+                    const int a = pr.x, b = pr.y;
+                    size_t k = 0;
+                    for (;; ++k) {
+                        if (k == taken.size()) taken.emplace_back(nb, 0);
+                        if (!taken[k][a] && !taken[k][b]) break;
+                    }
+                    taken[k][a] = taken[k][b] = 1;
+                    if (sets.size() <= k) sets.resize(k + 1);
+                    sets[k].push_back(make_int2((int)i, a | (b << 16)));
+                }
+            }
+            publish();
+            static const bool tdbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
+            if (tdbg) {
+                std::fprintf(stderr, "[jacobi] tail pass %d: %zu sets:", pass, sets.size());
+                for (const auto& st : sets) std::fprintf(stderr, " %zu", st.size());
+                std::fprintf(stderr, "\n");
+            }
+            if (sets.empty()) return;
+            Timed tm(c, "jacobi_round");
+            if (c->timing) c->timers["jacobi_round"].launches += (int)sets.size();
+            for (const auto& st : sets)
+                launch_jacobi_pairs(di, upload(c, st), (int)st.size(), block, c->jac_max_inner, c->stream);
+        }
+    };
     // Items still rotating at the sweep cap get a second window of the same length
     // before the call fails; the converged ones are skipped by every launch, so the
     // extra sweeps cost only the stragglers' work (and the launches' latency).
@@ -487,7 +661,6 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     // every later one: the rounds read their index as an offset from a counter in
     // device memory that k_conv_update advances by the sweep's length, so each replay
     // runs the next sweep's rounds (the same indices as direct launches).
-    static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
     const bool use_graph = c->use_graphs && !kdbg;
     int* round_base = use_graph ? remaining + 2 : nullptr;
     struct SweepGraph {
@@ -504,13 +677,13 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
             // one event pair per window of back-to-back rounds (per-launch events would
             // cost more host time than the short rounds themselves)
             Timed tm(c, "jacobi_round");
-            if (c->timing) c->timers["jacobi_round"].launches += rmax - 1;
+            if (c->timing) c->timers["jacobi_round"].launches += rmax;
             if (use_graph) {
                 if (!sg.exec) {
                     QH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                     for (int r = 0; r < rmax; ++r)
                         launch_jacobi_round(di, dt, (int)tiles.size(), r, round_base, block, c->jac_sort_sweeps, 0,
-                                            c->jac_max_inner, c->stream);
+                                            c->jac_max_inner, tail_switch, c->stream);
                     const hipError_t e1 = hipGetLastError();
                     const hipError_t e2 = hipStreamEndCapture(c->stream, &sg.graph);
                     QH_HIP(e1);
@@ -522,7 +695,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
                 for (int r = 0; r < rmax; ++r) {
                     const int g = sweep * rmax + r;
                     launch_jacobi_round(di, dt, (int)tiles.size(), g, nullptr, block, c->jac_sort_sweeps, kdbg,
-                                        c->jac_max_inner, c->stream);
+                                        c->jac_max_inner, tail_switch, c->stream);
                 }
             }
         }
@@ -531,35 +704,48 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
             readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
             std::fprintf(stderr, "[jacobi] sweep %d rmax %d rot:", sweep, rmax);
             for (size_t i = 0; i < items.size(); ++i)
-                std::fprintf(stderr, " %d/%d(%dx%d,nb%d)", hrot[2 * i], hrot[2 * i + 1], items[i].p, items[i].q,
-                             items[i].nb);
+                std::fprintf(stderr, " %d/%d/%d(%dx%d,nb%d)", hrot[CS * i], hrot[CS * i + 1], hrot[CS * i + 2],
+                             items[i].p, items[i].q, items[i].nb);
             std::fprintf(stderr, "\n");
         }
         if (c->timing) {
             readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
             auto& t = c->timers["jacobi_round"];
             for (size_t i = 0; i < items.size(); ++i) {
-                if (converged[i]) continue;
+                if (converged[i] || hrot[CS * i + 2] == 1) continue;
                 const double checked = (double)(items[i].nb / 2) * rmax;
-                t.flops += checked * gram_f[i] + hrot[2 * i] * upd_f[i];
-                t.bytes += checked * gram_b[i] + hrot[2 * i] * upd_b[i];
-                if (hrot[2 * i] == 0) converged[i] = 1;
+                t.flops += checked * gram_f[i] + hrot[CS * i] * upd_f[i];
+                t.bytes += checked * gram_b[i] + hrot[CS * i] * upd_b[i];
+                if (hrot[CS * i] == 0) converged[i] = 1;
             }
         }
-        launch_conv_update(di, (int)items.size(), remaining, round_base, rmax, c->stream);
+        launch_conv_update(di, (int)items.size(), remaining, round_base, rmax, sweep * rmax, c->jac_sort_sweeps,
+                           tail_switch, c->stream);
         c->timers["jacobi_sweep"].launches += 1;
-        int rem[2] = {0, 0};
+        int rem[4] = {0, 0, 0, 0};
         readback(c, rem, remaining, sizeof(rem));
         if (rem[1] > 0) {
-            std::vector<int> h(2 * items.size());
+            std::vector<int> h(CS * items.size());
             readback(c, h.data(), counters, h.size() * sizeof(int));
             std::string msg = "non-finite entries in the matrix to decompose:";
             for (size_t i = 0; i < items.size(); ++i)
-                if (h[2 * i + 1] < 0) msg += " item " + std::to_string(i);
+                if (h[CS * i + 1] < 0) msg += " item " + std::to_string(i);
             dfree(c, counters);
+        if (tail_flags) dfree(c, tail_flags);
             free_fro2();
             dfree(c, frob_part);
             throw Error(QH_ERR_NUMERIC, msg);
+        }
+        if (rem[3] > 0) {  // items waiting for the tail passes
+            run_tail((sweep + 1) * rmax);
+            readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
+            bool left = false;  // an item back in plain sweeps, or still rotating in them
+            for (size_t i = 0; i < items.size(); ++i) left = left || hrot[CS * i + 1] == 0;
+            if (!left) {
+                done = true;
+                break;
+            }
+            continue;
         }
         if (rem[0] == 0) {
             done = true;
@@ -567,12 +753,12 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         }
     }
     if (!done) {
-        std::vector<int> h(2 * items.size());
+        std::vector<int> h(CS * items.size());
         readback(c, h.data(), counters, h.size() * sizeof(int));
         std::string msg = "Jacobi SVD did not converge within " + std::to_string(hard_cap) + " sweeps:";
         if (const char* dump = std::getenv("QH_DEBUG_JACOBI_DUMP")) {
             for (size_t i = 0; i < items.size(); ++i)
-                if (!h[2 * i + 1]) {
+                if (!h[CS * i + 1]) {
                     std::vector<cplx> hx((size_t)items[i].p * items[i].q);
                     readback(c, hx.data(), items[i].x, hx.size() * sizeof(cplx));
                     if (FILE* f = std::fopen(dump, "wb")) {
@@ -586,17 +772,19 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         }
         int shown = 0;
         for (size_t i = 0; i < items.size() && shown < 4; ++i)
-            if (!h[2 * i + 1]) {
+            if (!h[CS * i + 1]) {
                 msg += " item " + std::to_string(i) + " (" + std::to_string(items[i].p) + "x" +
                        std::to_string(items[i].q) + ")";
                 ++shown;
             }
         dfree(c, counters);
+        if (tail_flags) dfree(c, tail_flags);
         free_fro2();
         dfree(c, frob_part);
         throw Error(QH_ERR_NOT_CONVERGED, msg);
     }
     dfree(c, counters);
+    if (tail_flags) dfree(c, tail_flags);
     free_fro2();
     dfree(c, frob_part);
 }

@@ -75,6 +75,7 @@ struct qh_context {
     int jac_precondition = 3;
     int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
     bool use_graphs = true;   // a Jacobi sweep's rounds captured once and replayed as a HIP graph
+    bool jac_tail = true;     // the last sweeps as whole-sweep checks plus launches of the flagged pairs
     bool pivot = true;        // first QR pass on the columns ordered by decreasing norm
     // set by qh_ctx_autotune: QR passes per size class (columns up to the key) and the
     // Jacobi block width per largest class of a batch; empty = the settings above
