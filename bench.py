@@ -316,6 +316,82 @@ def cpu_baseline_leg(sat_arrays, sat_lams, layer, trunc, n_updates, reps):
                       f"median of {reps} repetitions after one warm-up, all host threads and 1 thread"}
 
 
+def sample_legs(qh, wl, ctx, arrays, layers, trunc, args):
+    """The legs beside the timed steps on a saturated state of the same workload: the
+    batched FP64 sample, the FP32 / bf16x3 θ forms, the contractions, the per-call FP64
+    form and the CPU baseline.  Returns (extra fields, cpu_baseline)."""
+    # saturated sample: layers of Trotter steps 11.. (χ = 256 in the bulk)
+    sat_start = max(0, min(30, len(layers) - args.sample_layers))
+    sample = layers[sat_start: sat_start + args.sample_layers]
+    sample_updates = sum(len(l) for l in sample)
+    state_sat, _ = run_workload(qh, arrays, layers, trunc, stop_layer=sat_start)
+    sat_arrays = state_sat.internal_arrays()
+    sat_lams = state_sat.lambdas()
+
+    log("saturated state ready")
+    extra = {}
+    # batched on the sample
+    psi_b = state_sat.copy()
+    ctx.synchronize()
+    tb0 = time.perf_counter()
+    for layer in sample:
+        qh.evolve_layer_([(psi_b, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
+                         maxdim=trunc["maxdim"])
+    ctx.synchronize()
+    tb = time.perf_counter() - tb0
+    extra["saturated_batched_updates_per_s"] = round(sample_updates / tb, 2)
+    z_terms = [[qh.gate(wl.Z, i)] for i in range(1, psi_b.n + 1)]
+    z64 = qh.expect_many(psi_b, z_terms)
+    mixed = {}
+    for prec in ("fp32", "bf16x3"):
+        psi_m = state_sat.copy()
+        ctx.synchronize()
+        tm0 = time.perf_counter()
+        for layer in sample:
+            qh.evolve_layer_([(psi_m, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
+                             maxdim=trunc["maxdim"], precision=prec)
+        ctx.synchronize()
+        tm = time.perf_counter() - tm0
+        zm = qh.expect_many(psi_m, z_terms)
+        mixed[prec] = {"updates_per_s": round(sample_updates / tm, 2),
+                       "max_abs_dZ_vs_fp64": float(np.max(np.abs(zm - z64))),
+                       "sum_bond_dims": int(sum(psi_m.bond_dims())),
+                       "sum_bond_dims_fp64": int(sum(psi_b.bond_dims()))}
+    extra["mixed_precision_sample"] = mixed
+    extra["contractions"] = contractions_leg(qh, wl, psi_b)
+    log("contraction sample done")
+    if not args.no_plain:
+        psi_p = state_sat.copy()
+        ctx.synchronize()
+        tp0 = time.perf_counter()
+        for layer in sample:
+            for b, g in layer:
+                qh.evolve_(psi_p, qh.gate(g, b, b + 1), threshold=trunc["threshold"], maxdim=trunc["maxdim"])
+        ctx.synchronize()
+        tp = time.perf_counter() - tp0
+        extra["saturated_plain_fp64_updates_per_s"] = round(sample_updates / tp, 2)
+        extra["batched_over_plain"] = round(tp / tb, 3)
+        # the mixed-precision target: batched FP32-contraction updates against the plain
+        # (per-call, unbatched) FP64 form, at <= 1e-6 error in <Z_i>; and, separately,
+        # against the batched FP64 form (the SVD is FP64 in both, so this ratio is ~1)
+        m32 = mixed["fp32"]
+        extra["mixed_precision_target"] = {
+            "mixed_over_plain_fp64": round(m32["updates_per_s"] * tp / sample_updates, 3),
+            "mixed_over_batched_fp64": round(m32["updates_per_s"] * tb / sample_updates, 3),
+            "max_abs_dZ_vs_fp64": m32["max_abs_dZ_vs_fp64"], "bound": 1e-6, "speedup_goal": 5.0,
+            "met": bool(m32["updates_per_s"] * tp / sample_updates >= 5.0 and m32["max_abs_dZ_vs_fp64"] <= 1e-6),
+            "note": "speed from batching; the SVD stays FP64 (FP32 Jacobi sweeps cannot resolve the "
+                    "spectrum below ~1e-7 of sigma_1, where half the kept values of a saturated theta lie)"}
+        extra["saturated_sample"] = f"{args.sample_layers} layers ({sample_updates} updates) at chi<=256 after 10 Trotter steps"
+
+    log("plain sample done")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_leg(sat_arrays, sat_lams, sample[0], trunc, args.cpu_updates, args.cpu_reps)
+
+    return extra, cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +403,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU-baseline repetitions (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-samples", action="store_true",
+                    help="only the timed steps (no saturated sample, contraction, plain or CPU legs): "
+                         "a profile of the process then covers exactly the timed launches")
     ap.add_argument("--serving", choices=("sample", "full", "off"), default="full",
                     help="request-loop leg: the full serving distribution (default: 12 requests at 0.1 QPS), "
                          "a lighter sample (n <= 64, chimax <= 256), or none")
@@ -481,74 +560,9 @@ def main():
                   "frac": round(t_star / elapsed, 4),
                   "algorithmic_gflop_per_step": round((account["f_gemm"] + account["f_svd"]) / 1e9, 1)}
 
-    # saturated sample: layers of Trotter steps 11.. (χ = 256 in the bulk)
-    sat_start = max(0, min(30, len(layers) - args.sample_layers))
-    sample = layers[sat_start: sat_start + args.sample_layers]
-    sample_updates = sum(len(l) for l in sample)
-    state_sat, _ = run_workload(qh, arrays, layers, trunc, stop_layer=sat_start)
-    sat_arrays = state_sat.internal_arrays()
-    sat_lams = state_sat.lambdas()
-
-    log("saturated state ready")
-    extra = {}
-    # batched on the sample
-    psi_b = state_sat.copy()
-    ctx.synchronize()
-    tb0 = time.perf_counter()
-    for layer in sample:
-        qh.evolve_layer_([(psi_b, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
-                         maxdim=trunc["maxdim"])
-    ctx.synchronize()
-    tb = time.perf_counter() - tb0
-    extra["saturated_batched_updates_per_s"] = round(sample_updates / tb, 2)
-    z_terms = [[qh.gate(wl.Z, i)] for i in range(1, psi_b.n + 1)]
-    z64 = qh.expect_many(psi_b, z_terms)
-    mixed = {}
-    for prec in ("fp32", "bf16x3"):
-        psi_m = state_sat.copy()
-        ctx.synchronize()
-        tm0 = time.perf_counter()
-        for layer in sample:
-            qh.evolve_layer_([(psi_m, qh.gate(g, b, b + 1)) for b, g in layer], threshold=trunc["threshold"],
-                             maxdim=trunc["maxdim"], precision=prec)
-        ctx.synchronize()
-        tm = time.perf_counter() - tm0
-        zm = qh.expect_many(psi_m, z_terms)
-        mixed[prec] = {"updates_per_s": round(sample_updates / tm, 2),
-                       "max_abs_dZ_vs_fp64": float(np.max(np.abs(zm - z64))),
-                       "sum_bond_dims": int(sum(psi_m.bond_dims())),
-                       "sum_bond_dims_fp64": int(sum(psi_b.bond_dims()))}
-    extra["mixed_precision_sample"] = mixed
-    extra["contractions"] = contractions_leg(qh, wl, psi_b)
-    log("contraction sample done")
-    if not args.no_plain:
-        psi_p = state_sat.copy()
-        ctx.synchronize()
-        tp0 = time.perf_counter()
-        for layer in sample:
-            for b, g in layer:
-                qh.evolve_(psi_p, qh.gate(g, b, b + 1), threshold=trunc["threshold"], maxdim=trunc["maxdim"])
-        ctx.synchronize()
-        tp = time.perf_counter() - tp0
-        extra["saturated_plain_fp64_updates_per_s"] = round(sample_updates / tp, 2)
-        extra["batched_over_plain"] = round(tp / tb, 3)
-        # the mixed-precision target: batched FP32-contraction updates against the plain
-        # (per-call, unbatched) FP64 form, at <= 1e-6 error in <Z_i>; and, separately,
-        # against the batched FP64 form (the SVD is FP64 in both, so this ratio is ~1)
-        m32 = mixed["fp32"]
-        extra["mixed_precision_target"] = {
-            "mixed_over_plain_fp64": round(m32["updates_per_s"] * tp / sample_updates, 3),
-            "mixed_over_batched_fp64": round(m32["updates_per_s"] * tb / sample_updates, 3),
-            "max_abs_dZ_vs_fp64": m32["max_abs_dZ_vs_fp64"], "bound": 1e-6, "speedup_goal": 5.0,
-            "met": bool(m32["updates_per_s"] * tp / sample_updates >= 5.0 and m32["max_abs_dZ_vs_fp64"] <= 1e-6),
-            "note": "speed from batching; the SVD stays FP64 (FP32 Jacobi sweeps cannot resolve the "
-                    "spectrum below ~1e-7 of sigma_1, where half the kept values of a saturated theta lie)"}
-        extra["saturated_sample"] = f"{args.sample_layers} layers ({sample_updates} updates) at chi<=256 after 10 Trotter steps"
-
-    log("plain sample done")
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline_leg(sat_arrays, sat_lams, sample[0], trunc, args.cpu_updates, args.cpu_reps)
+    extra, cpu = {}, None
+    if not args.no_samples:
+        extra, cpu = sample_legs(qh, wl, ctx, arrays, layers, trunc, args)
 
     out = {
         "metric": "2-site gate+SVD updates/s at chi=256 (MPS TEBD, Ising Trotter workload)",
