@@ -31,6 +31,27 @@ __device__ __forceinline__ bool tail_due(const JacItem& it, int s, int sort_swee
     const long long cnt = it.own[w % 5];
     return cnt > 0 && 1000LL * cnt <= tail_switch * pairs;
 }
+// The round kernels' entry decision, taken by one thread per workgroup: whether the
+// workgroup runs (the item is not waiting for the tail passes) and, at the start of the
+// item's own sweep `own` (round % R == 0), the tail criterion on its previous sweeps
+// (every workgroup of the item reads the same counts and takes the same decision;
+// workgroup 0 records it) and the rotation count of the next sweep cleared.
+__device__ __forceinline__ int round_gate(const JacItem& it, int round, int own, int R, int pair, int sort_sweeps,
+                                          int tail_switch) {
+    const int ts = *((volatile int*)it.tail);
+    int go = ts != 1;
+    if (go && round % R == 0) {
+        if (ts == 0 && tail_due(it, own, sort_sweeps, tail_switch)) {
+            go = 0;
+            if (pair == 0) {
+                *it.tail = 1;
+                it.own[JAC_TAIL_AT] = own;
+            }
+        }
+        if (go && pair == 0) it.own[(own + 1) % 5] = 0;
+    }
+    return go;
+}
 // rr_pair(round + 1, k, nidx) from rr_pair(round, k, nidx)
 __device__ __forceinline__ void rr_next(int k, int nidx, int& a, int& b) {
     const int m = nidx - 1;
@@ -161,27 +182,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         own = round / R;
         sort = round < sort_sweeps * R;
         rr_pair(round % R, tt.y, it.nb, bi, bj);
-        // One thread reads the item's tail state (which the item's workgroup 0 may set
-        // during this launch) and decides for the workgroup.  At the start of the item's
-        // own sweep `own`: the tail criterion on the block pairs its previous sweeps
-        // rotated (every workgroup reads the same counts and takes the same decision), and
-        // the count of the next sweep cleared.  Counted per own sweep, the decision does
-        // not depend on the rest of the batch.
-        if (tid == 0) {
-            const int ts = *((volatile int*)it.tail);
-            int go = ts != 1;
-            if (go && round % R == 0) {
-                if (ts == 0 && tail_due(it, own, sort_sweeps, tail_switch)) {
-                    go = 0;
-                    if (tt.y == 0) {
-                        *it.tail = 1;
-                        it.own[JAC_TAIL_AT] = own;
-                    }
-                }
-                if (go && tt.y == 0) it.own[(own + 1) % 5] = 0;
-            }
-            sGo = go;
-        }
+        // (the item's tail state may be set by its workgroup 0 during this launch;
+        // counted per own sweep, the decision does not depend on the rest of the batch)
+        if (tid == 0) sGo = round_gate(it, round, own, R, tt.y, sort_sweeps, tail_switch);
     }
     if (tid < BC) {
         const int c = (tid < B) ? bi * B + tid : bj * B + (tid - B);
@@ -670,7 +673,7 @@ __device__ __forceinline__ void gram64_store(double2 (*sG)[65], const f64x4 (&P)
 __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restrict__ items,
                                                         const int2* __restrict__ tiles, int round_arg,
                                                         const int* __restrict__ round_base, int sort_sweeps,
-                                                        int max_inner) {
+                                                        int max_inner, int mode, int tail_switch) {
     constexpr int BC = 64, B = 32, LP = BC + 1, CH = 32, NP = 32, NT = 512;
     __shared__ double2 sG[BC][LP];
     __shared__ double2 sQ[BC][LP];
@@ -678,6 +681,7 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
     __shared__ int sPerm[BC], sSlot[BC], sOrd[BC];
     __shared__ double sKey[BC];
     __shared__ int sFlag;
+    __shared__ int sGo;
     // the Gram's 32-row chunks are staged in sQ's storage (Q is initialised afterwards)
     static_assert(BC * (CH + 1) <= BC * LP, "staging fits in sQ");
     double2 (*sX)[CH + 1] = reinterpret_cast<double2 (*)[CH + 1]>(&sQ[0][0]);
@@ -686,18 +690,28 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
     const int2 tt = tiles[blockIdx.x];
     const JacItem it = items[tt.x];
     if (*((volatile int*)it.conv)) return;
-    const int round = round_base ? *round_base + round_arg : round_arg;
     const int R = it.nb - 1;
-    const int r = round % R;
     const double tol2 = it.tol * it.tol;
-    const int sort = round < sort_sweeps * R;
-    int bi, bj;
-    rr_pair(r, tt.y, it.nb, bi, bj);
+    // the modes of k_jacobi_round: a tournament round (with the item's round offset and
+    // tail gate), or an explicit block pair of the tail passes (tile: item, bi | bj << 16)
+    int round = -1, sort = 0, bi, bj, own = 0;
+    if (tid == 0) sGo = 1;
+    if (mode == JAC_PAIRS) {
+        bi = tt.y & 0xffff;
+        bj = tt.y >> 16;
+    } else {
+        round = (round_base ? *round_base + round_arg : round_arg) - it.own[JAC_ROUND_OFF];
+        own = round / R;
+        sort = round < sort_sweeps * R;
+        rr_pair(round % R, tt.y, it.nb, bi, bj);
+        if (tid == 0) sGo = round_gate(it, round, own, R, tt.y, sort_sweeps, tail_switch);
+    }
     if (tid < BC) {
         const int c = (tid < B) ? bi * B + tid : bj * B + (tid - B);
         sCol[tid] = (c < it.q) ? c : -1;
     }
     __syncthreads();
+    if (!sGo) return;
     const int p = it.p;
     const double gfloor = it.floor_rel2 * (*it.fro2);
     const int lane = tid & 63, wv = tid >> 6;
@@ -857,7 +871,10 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
         const int any = sFlag;
         __syncthreads();
         if (!any) break;
-        if (sweep == 0 && tid == 0) atomicAdd(it.rot, 1);
+        if (sweep == 0 && tid == 0) {  // counted only if a rotation happened
+            atomicAdd(it.rot, 1);
+            if (mode == JAC_ROUND) atomicAdd(&it.own[own % 5], 1);
+        }
     }
 
     // ---------------- order eigenpairs (first sweeps only; as for block width 16) ----------------
@@ -969,7 +986,8 @@ void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles
                                                          max_inner, JAC_ROUND, nullptr, nullptr,
                                                          tail_switch);
     else if (block == 32)
-        k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, max_inner);
+        k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, max_inner,
+                                                JAC_ROUND, tail_switch);
     else
         k_jacobi_round<32, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug,
                                                          max_inner, JAC_ROUND, nullptr, nullptr,
@@ -985,19 +1003,19 @@ void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles
         k_jacobi_round<32, true><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, 1, JAC_CHECK, d_flags, d_flag0,
                                                         0);
 }
-// Convergence test of whole items without rotating (block width 16): G = X^H X in
-// 64 × 64 tiles of the upper block triangle, one workgroup per tile, on the matrix cores
-// (v_mfma_f64_16x16x4, three real products per complex block, as in the rounds' Gram);
-// wave w holds the four 16 × 16 sub-tiles of block row w of the tile, i.e. the block
-// pairs (I, J) = (4·ti + w, 4·tj + b).  The column norms come from the same LDS chunks.
-// flags[first[item] + I·nb + J] = 1 where a column pair of blocks I <= J fails the
-// rotation test (for I = J the pairs inside the block), 0 otherwise; the blocks past the
-// item and the lower half of a diagonal tile are not written.  Rows are streamed in
+// Convergence test of whole items without rotating (block width 16 or 32): G = X^H X in
+// 64 × 64 tiles of the upper triangle of 16-column blocks, one workgroup per tile, on the
+// matrix cores (v_mfma_f64_16x16x4, three real products per complex block, as in the
+// rounds' Gram); wave w holds the four 16 × 16 sub-tiles of row w of the tile, i.e. the
+// 16-column block pairs (4·ti + w, 4·tj + b).  The column norms come from the same LDS
+// chunks.  flags[first[item] + I·nb + J] is set to 1 where a column pair of the item's
+// blocks I <= J (bsub = block / 16 sixteen-column blocks each) fails the rotation test
+// (for I = J the pairs inside the block); the table starts at zero.  Rows are streamed in
 // chunks of 16, the next chunk fetched while the current one is multiplied.
 constexpr int JCT = 64, JCR = 16;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jacobi_check_tiles(const JacItem* __restrict__ items,
                                                             const int4* __restrict__ tiles, int* __restrict__ flags,
-                                                            const int* __restrict__ first) {
+                                                            const int* __restrict__ first, int bsub) {
     __shared__ double2 sX[2 * JCT][JCR + 1];
     __shared__ double sN[2 * JCT];
     const int4 tt = tiles[blockIdx.x];
@@ -1055,11 +1073,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double tol2 = it.tol * it.tol;
     const double gfloor = it.floor_rel2 * (*it.fro2);
     const bool diag = tt.y == tt.z;
+    const int n16 = it.nb * bsub;  // 16-column blocks of the item
     const int I = 4 * tt.y + wv;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const int J = 4 * tt.z + b;
-        if (I >= it.nb || J >= it.nb || (diag && b < wv)) continue;  // wave-uniform
+        if (I >= n16 || J >= n16 || (diag && b < wv)) continue;  // wave-uniform
         bool bad = false;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -1069,18 +1088,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             bad = bad || needs_rotation(sN[16 * wv + i], sN[JCT + 16 * b + j], g, tol2, gfloor);
         }
         const bool any = __ballot(bad) != 0;
-        if (lane == 0) flags[first[tt.x] + I * it.nb + J] = any ? 1 : 0;
+        if (lane == 0 && any) flags[first[tt.x] + (I / bsub) * it.nb + J / bsub] = 1;
     }
 }
 void launch_jacobi_check_tiles(const JacItem* d_items, const int4* d_tiles, int ntiles, int* d_flags,
-                               const int* d_first, hipStream_t s) {
+                               const int* d_first, int block, hipStream_t s) {
     if (ntiles <= 0) return;
-    k_jacobi_check_tiles<<<ntiles, 256, 0, s>>>(d_items, d_tiles, d_flags, d_first);
+    k_jacobi_check_tiles<<<ntiles, 256, 0, s>>>(d_items, d_tiles, d_flags, d_first, block / 16);
 }
 void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles, int block, int max_inner,
                          hipStream_t s) {
     if (ntiles <= 0) return;
-    if (block == 8)
+    if (block == 32)
+        k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, max_inner, JAC_PAIRS, 0);
+    else if (block == 8)
         k_jacobi_round<16, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, max_inner, JAC_PAIRS,
                                                          nullptr, nullptr, 0);
     else

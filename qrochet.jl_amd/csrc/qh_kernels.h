@@ -122,7 +122,7 @@ void launch_frob2(const JacItem* d_items, int count, const int2* d_tiles, int nt
 constexpr int FROB_CHUNK = 65536;
 // block = columns per block (8 or 16); a workgroup handles 2·block columns.
 // round: the tournament round, plus *round_base when round_base is not null
-// tail_switch > 0 (per mille, block 8 and 16): at the start of each of its own sweeps after
+// tail_switch > 0 (per mille): at the start of each of its own sweeps after
 // the sorting ones, an item whose previous sweep rotated at most that share of its block
 // pairs stops (tail = 1) and waits for the tail passes (JAC_CHECK / JAC_PAIRS launches)
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
@@ -139,7 +139,7 @@ __host__ __device__ inline void rr_pair(int round, int k, int nidx, int& a, int&
         b = (round - k + m) % m;
     }
 }
-// Modes of the block-8/16 round kernel: a tournament round; the test of every (round,
+// Modes of the round kernels (JAC_CHECK at block width 8 only): a tournament round; the test of every (round,
 // pair) of a sweep in one launch, without rotating (tile: item, round · nb/2 + pair;
 // flags[flag0[item] + tile.y] = 1 where the pair needs a rotation); explicit block
 // pairs (tile: item, bi | bj << 16), which must be disjoint within an item.
@@ -148,11 +148,12 @@ void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles
                          int block, hipStream_t s);
 void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles, int block, int max_inner,
                          hipStream_t s);
-// block width 16: the test of every column pair of whole items (tile: item, ti, tj with
-// ti <= tj, 64-column tiles of the upper block triangle); flags[first[item] + I·nb + J]
-// for the blocks I <= J of the tile
+// block width 16 or 32: the test of every column pair of whole items (tile: item, ti, tj
+// with ti <= tj, 64-column tiles of the upper triangle of 16-column blocks); sets
+// flags[first[item] + I·nb + J] = 1 for the failing blocks I <= J (the table must start
+// at zero)
 void launch_jacobi_check_tiles(const JacItem* d_items, const int4* d_tiles, int ntiles, int* d_flags,
-                               const int* d_first, hipStream_t s);
+                               const int* d_first, int block, hipStream_t s);
 // V <- I, counters cleared (tile map: item, column)
 void launch_set_identity_items(const JacItem* d_items, const int2* d_tiles, int ntiles, hipStream_t s);
 // converged <- (no rotation in the window, outside the tail); remaining[0] = items still

@@ -500,13 +500,13 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     constexpr int TAIL_SWITCH = 700, TAIL_MAX = 250;  // per mille of an item's block pairs
     constexpr int TAIL_PASSES = 8;
     static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
-    const int tail_switch = (c->jac_tail && block != 32 && !kdbg) ? TAIL_SWITCH : 0;
+    const int tail_switch = (c->jac_tail && !kdbg) ? TAIL_SWITCH : 0;
     int* tail_flags = nullptr;
     std::vector<int> tail_first(items.size());
     size_t tail_total = 0;
-    // the test: by 64-column tiles of X^H X at block width 16 (flags per block pair I, J of
-    // an nb × nb table), per (round, pair) of a sweep otherwise
-    const bool tile_check = block == 16;
+    // the test: by 64-column tiles of X^H X at block widths 16 and 32 (flags per block pair
+    // I, J of an nb × nb table), per (round, pair) of a sweep at width 8
+    const bool tile_check = block != 8;
     for (size_t i = 0; i < items.size(); ++i) {
         tail_first[i] = (int)tail_total;
         const size_t nb = items[i].nb;
@@ -550,7 +550,7 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
                 }
                 const int nb = items[i].nb;
                 if (tile_check) {
-                    const int nt = (nb + 3) / 4;
+                    const int nt = (nb * (block / 16) + 3) / 4;  // 64-column tiles
                     for (int ti = 0; ti < nt; ++ti)
                         for (int tj = ti; tj < nt; ++tj) ttiles.push_back(make_int4((int)i, ti, tj, 0));
                 } else {
@@ -571,12 +571,14 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
             }
             {
                 Timed tm(c, "jacobi_check");
-                if (tile_check)
-                    launch_jacobi_check_tiles(di, upload(c, ttiles), (int)ttiles.size(), tail_flags, d_first,
+                if (tile_check) {
+                    QH_HIP(hipMemsetAsync(tail_flags, 0, tail_total * sizeof(int), c->stream));
+                    launch_jacobi_check_tiles(di, upload(c, ttiles), (int)ttiles.size(), tail_flags, d_first, block,
                                               c->stream);
-                else
+                } else {
                     launch_jacobi_check(di, upload(c, ctiles), (int)ctiles.size(), tail_flags, d_first, block,
                                         c->stream);
+                }
             }
             readback(c, hflags.data(), tail_flags, tail_total * sizeof(int));
             // per set: the tiles of the flagged pairs; per item and set: the blocks taken
@@ -589,8 +591,8 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
                     c->timers["jacobi_check"].flops += cnt * gram_f[i];
                     c->timers["jacobi_check"].bytes += cnt * gram_b[i];
                 }
-                // the flagged block pairs, in round order; at width 16 a block with a pair
-                // inside it failing and in no flagged pair rotates with its neighbour I ^ 1
+                // the flagged block pairs, in round order; from the tiled test, a block with a
+                // pair inside it failing and in no flagged pair rotates with its neighbour I ^ 1
                 std::vector<int2> fl;
                 const int* hf = &hflags[tail_first[i]];
                 if (tile_check) {
