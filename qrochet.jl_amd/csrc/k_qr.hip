@@ -278,13 +278,13 @@ __device__ void qr_panel_global(const QrItem& it, int j0, PanelShared& sh) {
     }
 }
 
-// Panel factorisation with the panel in registers, for panels of up to 2·NT rows:
-// thread tid owns rows tid and tid + NT of the (rows × QR_NB) panel, so the sums and
+// Panel factorisation with the panel in registers, for panels of up to RPT·NT rows:
+// thread tid owns rows tid + j·NT (j < RPT) of the (rows × QR_NB) panel, so the sums and
 // the trailing update of a column read no memory.  The column index is a template
 // parameter (one instantiation per column), so every register index is static.  Same
 // arithmetic, in the same order, as qr_panel_columns.
-template <int C, int NT>
-__device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, int nb, PanelShared& sh) {
+template <int C, int NT, int RPT>
+__device__ __forceinline__ void qr_reg_column(double2 (&P)[RPT][QR_NB], int rows, int nb, PanelShared& sh) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x;
     double2 sk[QR_NB];
@@ -293,7 +293,7 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     // (x is 0 outside the rows below the diagonal, and the columns past the panel are 0,
     // so every product is formed without a branch)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < RPT; ++j) {
         const int r = tid + NT * j;
         const double2 x = (r > C && r < rows) ? P[j][C] : cz();
 #pragma unroll
@@ -312,9 +312,9 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     const double2 dl = panel_coef(tid & 15, C, tot, sh.sRow[C & 1][tid & 15], h);
     const double2* dk = panel_dots_publish(dl, sh);
     // v on the owned rows (0 on rows above the diagonal and past the panel)
-    double2 vr[2];
+    double2 vr[RPT];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < RPT; ++j) {
         const int r = tid + NT * j;
         vr[j] = cz();
         if (r == C) vr[j] = make_double2(1.0, 0.0);
@@ -347,32 +347,33 @@ __device__ __forceinline__ void qr_reg_column(double2 (&P)[2][QR_NB], int rows, 
     for (int k = C + 1; k < QR_NB; ++k) {
         const double2 wk = dk[k];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) P[j][k] = cfms(wk, vr[j], P[j][k]);
+        for (int j = 0; j < RPT; ++j) P[j][k] = cfms(wk, vr[j], P[j][k]);
     }
     if constexpr (C + 1 < QR_NB) {
-        if (C + 1 < nb) qr_reg_column<C + 1, NT>(P, rows, nb, sh);
+        if (C + 1 < nb) qr_reg_column<C + 1, NT, RPT>(P, rows, nb, sh);
     }
 }
 
-template <int NT>
+// RPT rows per thread: panels of up to RPT·NT rows
+template <int NT, int RPT = 2>
 __device__ __forceinline__ void qr_panel_regs(const QrItem& it, int j0, PanelShared& sh) {
     const int tid = threadIdx.x;
     const int rows = it.m - j0;
     const int nb = min(QR_NB, it.n - j0);
     const size_t lda = it.lda;
     const double2* a0 = it.a + j0 + lda * j0;
-    double2 P[2][QR_NB];
+    double2 P[RPT][QR_NB];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < RPT; ++j) {
         const int r = tid + NT * j;
 #pragma unroll
         for (int k = 0; k < QR_NB; ++k) P[j][k] = ld_or_zero(a0, r + lda * k, r < rows && k < nb);
     }
-    qr_reg_column<0, NT>(P, rows, nb, sh);
+    qr_reg_column<0, NT, RPT>(P, rows, nb, sh);
     __syncthreads();
     // R (upper part) back to a, the explicit reflectors to v
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < RPT; ++j) {
         const int r = tid + NT * j;
         if (r >= rows) continue;
 #pragma unroll
@@ -427,25 +428,28 @@ __global__ __launch_bounds__(512) void k_qr_panel_tall(const QrItem* __restrict_
 //   Y -= V W2 as (V W2)^T = W2^T V^T, so that a lane's results lie along one column
 //     of Y: A[j][k] = W2(k, j), B[k][i] = V(r0 + i, j0 + k); one 16-row block per
 //     step, wave wv takes blocks wv, wv + 4, ..., two at a time.
+template <int NW = 4>  // waves of the workgroup
 struct ApplyMfmaShared {
-    double2 sPart[4][QR_NB][QR_TC];
+    double2 sPart[NW][QR_NB][QR_TC];
     double2 sW[QR_NB][QR_TC + 1];
     double2 sW2[QR_NB][QR_TC + 1];
     double2 sT[QR_NB][QR_NB + 1];
 };
 typedef double qr_f64x4 __attribute__((ext_vector_type(4)));
+template <int NW = 4>  // waves of the workgroup (4 or 8)
 __device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c0, int tc, int j0, int nb,
-                                                    int adjoint, ApplyMfmaShared& sh) {
+                                                    int adjoint, ApplyMfmaShared<NW>& sh) {
     static_assert(QR_TC == QR_NB && QR_NB == 16, "tiles are 16 reflectors × 16 columns");
+    constexpr int NT = 64 * NW;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int m = it.m;
     const size_t ldy = it.ldy;
     const double2* t = it.t + (size_t)(j0 / QR_NB) * QR_NB * QR_NB;
-    for (int e = tid; e < QR_NB * QR_NB; e += 256) sh.sT[e % QR_NB][e / QR_NB] = gload(t, e);
+    for (int e = tid; e < QR_NB * QR_NB; e += NT) sh.sT[e % QR_NB][e / QR_NB] = gload(t, e);
     // ---- phase 1: W = V^H Y over rows j0 .. m - 1
     const int steps = (m - j0 + 3) / 4;
-    const int mine = (steps - wv + 3) / 4;  // this wave's steps: s = wv + 4 t, t < mine
+    const int mine = (steps - wv + NW - 1) / NW;  // this wave's steps: s = wv + NW t, t < mine
     const double2* vcol = it.v + (size_t)m * (j0 + li);
     const double2* ycol = it.y + ldy * (c0 + li);
     const bool vok = li < nb, yok = li < tc;
@@ -454,7 +458,7 @@ __device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c
     auto fetch = [&](int t0, double2 (&a)[U], double2 (&b)[U]) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int r = j0 + 4 * (wv + 4 * (t0 + u)) + lk;
+            const int r = j0 + 4 * (wv + NW * (t0 + u)) + lk;
             const bool ok = t0 + u < mine && r < m;
             a[u] = ld_or_zero(vcol, r, ok && vok);
             b[u] = ld_or_zero(ycol, r, ok && yok);
@@ -480,28 +484,34 @@ __device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c
     for (int r = 0; r < 4; ++r)  // conj(a) b = (P + Q) + i (S - P + Q)
         sh.sPart[wv][lk + 4 * r][li] = make_double2(p[r] + q[r], s[r] - p[r] + q[r]);
     __syncthreads();
-    const int wa = tid / QR_TC, wc = tid % QR_TC;
-    sh.sW[wa][wc] = cadd(cadd(sh.sPart[0][wa][wc], sh.sPart[1][wa][wc]),
-                         cadd(sh.sPart[2][wa][wc], sh.sPart[3][wa][wc]));
+    const int wa = (tid / QR_TC) % QR_NB, wc = tid % QR_TC;  // threads 0..255: one entry each
+    if (tid < QR_NB * QR_TC) {
+        double2 w = sh.sPart[0][wa][wc];
+#pragma unroll
+        for (int k = 1; k < NW; ++k) w = cadd(w, sh.sPart[k][wa][wc]);
+        sh.sW[wa][wc] = w;
+    }
     __syncthreads();
     // ---- phase 2: W2 = T' W
-    double2 w2 = cz();
-    for (int b = 0; b < QR_NB; ++b) {
-        const double2 tab = adjoint ? cconj(sh.sT[b][wa]) : sh.sT[wa][b];
-        w2 = cfma(tab, sh.sW[b][wc], w2);
+    if (tid < QR_NB * QR_TC) {
+        double2 w2 = cz();
+        for (int b = 0; b < QR_NB; ++b) {
+            const double2 tab = adjoint ? cconj(sh.sT[b][wa]) : sh.sT[wa][b];
+            w2 = cfma(tab, sh.sW[b][wc], w2);
+        }
+        sh.sW2[wa][wc] = w2;
     }
-    sh.sW2[wa][wc] = w2;
     __syncthreads();
     // ---- phase 3: Y -= V W2, 16-row blocks
     double2 wt[4];
 #pragma unroll
     for (int k4 = 0; k4 < 4; ++k4) wt[k4] = sh.sW2[4 * k4 + lk][li];
     const int nblk = (m - j0 + 15) / 16;
-    for (int blk0 = wv; blk0 < nblk; blk0 += 8) {
+    for (int blk0 = wv; blk0 < nblk; blk0 += 2 * NW) {
         double2 vv[2][4], yo[2][4];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int blk = blk0 + 4 * h;
+            const int blk = blk0 + NW * h;
             const int row = j0 + 16 * blk + li;
             const bool rok = blk < nblk && row < m;
 #pragma unroll
@@ -521,7 +531,7 @@ __device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c
                 dq = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, dq, 0, 0, 0);
                 ds = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, ds, 0, 0, 0);
             }
-            const int blk = blk0 + 4 * h;
+            const int blk = blk0 + NW * h;
             const int row = j0 + 16 * blk + li;
             if (blk < nblk && row < m) {
 #pragma unroll
@@ -539,7 +549,7 @@ __device__ __forceinline__ void qr_apply_panel_mfma(const QrApplyItem& it, int c
 // one block reflector (panel j0) on one column tile per workgroup
 __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles,
                                                   int j0, int adjoint) {
-    __shared__ ApplyMfmaShared sh;
+    __shared__ ApplyMfmaShared<4> sh;
     const int2 tt = tiles[blockIdx.x];
     const QrApplyItem it = items[tt.x];
     const int nb = min(QR_NB, it.nrefl - j0);
@@ -553,7 +563,7 @@ __global__ __launch_bounds__(256) void k_qr_apply(const QrApplyItem* __restrict_
 // Y <- Q Y with every block reflector of the factorisation, last panel first: the
 // column tiles are independent, so one workgroup walks all panels of its tile.
 __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__ items, const int2* __restrict__ tiles) {
-    __shared__ ApplyMfmaShared sh;
+    __shared__ ApplyMfmaShared<4> sh;
     const int2 tt = tiles[blockIdx.x];
     const QrApplyItem it = items[tt.x];
     const int c0 = tt.y * QR_TC;
@@ -570,23 +580,24 @@ __global__ __launch_bounds__(256) void k_qr_form(const QrApplyItem* __restrict__
 // others apply panel j0's block reflector to the trailing tiles beyond it.  The two
 // parts touch disjoint columns and disjoint reflector storage, so they run in one
 // launch and the panel's latency hides behind the trailing update.
-// (two waves per SIMD: the trailing-update workgroups need the occupancy)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_qr_step(
-    const QrItem* __restrict__ items, int count,
-                                                 const QrApplyItem* __restrict__ app, const int2* __restrict__ tiles,
-                                                 int j0) {
+// 512 threads (eight waves, two per SIMD): the panel's column steps run as two waves per
+// SIMD with one row per thread, and each trailing tile is applied by eight waves.
+constexpr int QR_STEP_NT = 512;
+__global__ __launch_bounds__(QR_STEP_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_qr_step(
+    const QrItem* __restrict__ items, int count, const QrApplyItem* __restrict__ app, const int2* __restrict__ tiles,
+    int j0) {
     __shared__ union {
         PanelShared p;
-        ApplyMfmaShared a;
+        ApplyMfmaShared<QR_STEP_NT / 64> a;
     } sh;
     if ((int)blockIdx.x < count) {
         const QrItem it = items[blockIdx.x];
         const int jn = j0 + QR_NB;
         if (jn >= it.n) return;
         const QrApplyItem ai = app[blockIdx.x];  // the same item's reflectors
-        qr_apply_panel_mfma(ai, jn, min(QR_TC, ai.ncols - jn), j0, QR_NB, 1, sh.a);
+        qr_apply_panel_mfma<QR_STEP_NT / 64>(ai, jn, min(QR_TC, ai.ncols - jn), j0, QR_NB, 1, sh.a);
         __syncthreads();
-        qr_panel_regs<256>(it, jn, sh.p);  // at most 512 rows (host routing)
+        qr_panel_regs<QR_STEP_NT, 1>(it, jn, sh.p);  // at most 512 rows (host routing)
         return;
     }
     const int2 tt = tiles[blockIdx.x - count];
@@ -596,7 +607,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int c0 = j0 + nb + tt.y * QR_TC;
     const int tc = min(QR_TC, ai.ncols - c0);
     if (tc <= 0) return;
-    qr_apply_panel_mfma(ai, c0, tc, j0, nb, 1, sh.a);
+    qr_apply_panel_mfma<QR_STEP_NT / 64>(ai, c0, tc, j0, nb, 1, sh.a);
 }
 
 void launch_qr_panel(const QrItem* d_items, int count, int j0, hipStream_t s) {
@@ -618,7 +629,7 @@ void launch_qr_form(const QrApplyItem* d_items, const int2* d_tiles, int ntiles,
 void launch_qr_step(const QrItem* d_items, int count, const QrApplyItem* d_app, const int2* d_tiles, int ntiles,
                     int j0, hipStream_t s) {
     if (count + ntiles <= 0) return;
-    k_qr_step<<<count + ntiles, 256, 0, s>>>(d_items, count, d_app, d_tiles, j0);
+    k_qr_step<<<count + ntiles, QR_STEP_NT, 0, s>>>(d_items, count, d_app, d_tiles, j0);
 }
 int qr_tile_cols() { return QR_TC; }
 

@@ -854,8 +854,9 @@ void run_qr_factor(qh_context* c, const std::vector<QrJob>& jobs) {
     int2* dr = upload(c, rest_tiles);
     Timed tm(c, "qr", flops, 0.0);
     if (mmax > 512) {
-        // panels of up to 1024 rows in registers need 512-thread workgroups, which the
-        // fused look-ahead step (256 threads) does not have: panel, then trailing update
+        // panels of up to 1024 rows in registers need two rows per thread, which the
+        // fused look-ahead step (one row per thread of 512) does not hold: panel, then
+        // trailing update
         for (int j0 = 0; j0 < nmax; j0 += QR_NB) {
             launch_qr_panel_tall(di, (int)items.size(), j0, c->stream);
             launch_qr_apply(da, dn, (int)next_tiles.size(), j0, 1, c->stream);
