@@ -141,8 +141,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     constexpr int RB = 128, IC = 8;
     constexpr int SX_ELEMS = (BC * (CH + 1) > IC * (RB + 1)) ? BC * (CH + 1) : IC * (RB + 1);
     __shared__ double2 sG[BC][LP];
-    __shared__ double2 sQ[BC][LP];
-    __shared__ double2 sXraw[SX_ELEMS];
+    // Q, then the staging area (the MFMA Gram stages its 64-row chunks through both:
+    // Q is initialised after the Gram)
+    __shared__ double2 sQX[BC * LP + SX_ELEMS];
+    double2 (*sQ)[LP] = reinterpret_cast<double2 (*)[LP]>(sQX);
+    double2* sXraw = sQX + BC * LP;
     double2 (*sX)[CH + 1] = reinterpret_cast<double2 (*)[CH + 1]>(sXraw);
     __shared__ int sCol[BC];
     __shared__ int sPerm[BC];
@@ -232,20 +235,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // and one LDS value per lane gives the operands of X_I^H X_J (A[i][k] = x_i at
         // row k, B[k][j] = x_j at row k).  Three real products per complex block:
         // P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi); Re = P1 + P2, Im = P3 - P1 + P2
-        // (the conjugate of A is in the signs).  Wave wv takes rows wv + 4t (t < 8) of
-        // each 32-row chunk, four per step; result register q of lane l is
-        // G[l / 16 + 4 q][l % 16].
+        // (the conjugate of A is in the signs).  Rows are staged in 64-row chunks through
+        // the storage of Q and the staging area, the next chunk fetched (unconditional
+        // loads: a row past the end or an absent column reads an element in range,
+        // zeroed when stashed, so no branch holds a load back) while the current one is
+        // multiplied.  Wave wv takes rows wv + 4t (t < 16) of each chunk, four per step;
+        // result register q of lane l is G[l / 16 + 4 q][l % 16].
+        constexpr int CG = 64, LPG = BC * CG / 256;
+        static_assert(BC * (CG + 1) <= BC * LP + SX_ELEMS, "a 64-row chunk fits in Q and the staging area");
+        double2 (*sXg)[CG + 1] = reinterpret_cast<double2 (*)[CG + 1]>(sQX);
+        double2 pg[LPG];
+        auto fetch_to = [&](int row0) {
+#pragma unroll
+            for (int s2 = 0; s2 < LPG; ++s2) {
+                const int e = tid + 256 * s2;
+                const int c = e / CG, rr = e % CG;
+                pg[s2] = gload(it.x, (size_t)min(row0 + rr, p - 1) + (size_t)p * max(sCol[c], 0));
+            }
+        };
+        auto stash_from = [&](int row0) {
+#pragma unroll
+            for (int s2 = 0; s2 < LPG; ++s2) {
+                const int e = tid + 256 * s2;
+                const int c = e / CG, rr = e % CG;
+                sXg[c][rr] = (sCol[c] >= 0 && row0 + rr < p) ? pg[s2] : cz();
+            }
+        };
         f64x4 p11 = {0.0, 0.0, 0.0, 0.0}, q11 = p11, s11 = p11, p12 = p11, q12 = p11, s12 = p11;
         f64x4 p22 = p11, q22 = p11, s22 = p11;
-        fetch(it.x, p, 0);
-        for (int row0 = 0; row0 < p; row0 += CH) {
-            stash();
-            __syncthreads();
-            if (row0 + CH < p) fetch(it.x, p, row0 + CH);
+        auto reduce_chunk = [&]() {
 #pragma unroll
-            for (int s4 = 0; s4 < CH / 16; ++s4) {
+            for (int s4 = 0; s4 < CG / 16; ++s4) {
                 const int rr = wv + 4 * (4 * s4 + (lane >> 4));
-                const double2 x1 = sX[lane & 15][rr], x2 = sX[B + (lane & 15)][rr];
+                const double2 x1 = sXg[lane & 15][rr], x2 = sXg[B + (lane & 15)][rr];
                 const double d1 = x1.x - x1.y, e1 = x1.x + x1.y, e2 = x2.x + x2.y, d2 = x2.x - x2.y;
                 p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, x1.x, p11, 0, 0, 0);
                 q11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, x1.y, q11, 0, 0, 0);
@@ -257,6 +279,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 q22 = __builtin_amdgcn_mfma_f64_16x16x4f64(x2.y, x2.y, q22, 0, 0, 0);
                 s22 = __builtin_amdgcn_mfma_f64_16x16x4f64(d2, e2, s22, 0, 0, 0);
             }
+        };
+        fetch_to(0);
+        for (int row0 = 0; row0 < p; row0 += CG) {
+            stash_from(row0);
+            __syncthreads();
+            if (row0 + CG < p) fetch_to(row0 + CG);
+            reduce_chunk();
             __syncthreads();
         }
         for (int w = 0; w < 4; ++w) {
@@ -497,8 +526,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __builtin_amdgcn_s_setprio(0);
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q ----------------
-    // 128-row panels.  BC = 32 runs on the matrix cores with its operands loaded
-    // directly (below).  Otherwise register-blocked on the vector ALUs: thread (rg, cg)
+    // BC = 32 runs on the matrix cores with its operands loaded directly (below).
+    // Otherwise register-blocked on the vector ALUs in 128-row panels: thread (rg, cg)
     // accumulates rows rg + 32a (a < RA) × columns cg·JG + b (b < JG) of a panel, so
     // every LDS value feeds RA or JG complex FMAs; the panel's X is staged IC inner
     // columns at a time (prefetched into registers one stage ahead); the sum runs over
@@ -524,77 +553,87 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             sU[e / RB][e % RB] = upre[s2];
         }
     };
+    if constexpr (BC == 32) {
+        // On the matrix cores (v_mfma_f64_16x16x4), in 16-row tiles: wave mw takes rows
+        // [64 k + 16 mw, 64 k + 16 mw + 16) of X (k < kx) and then of V, times Q's two
+        // 16-column tiles.  A complex tile product is three real ones: P1 = Xr Qr,
+        // P2 = Xi Qi, P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.  Operand
+        // lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l is
+        // D[l / 16 + 4 q][l % 16].
+        // A operands straight from HBM: the wave reads and writes only its own rows (16
+        // consecutive rows of one column per 16 lanes), so the update needs no staging
+        // and no barrier; Q is read-only in LDS after the eigensolver.  The next tile's
+        // loads are issued before the current tile's products, so they arrive while the
+        // matrix cores work.  The loads are unconditional: a row past the end or an
+        // absent column reads an element in range instead -- absent columns meet zero
+        // rows of Q (their positions keep e_j), rows past the end are not stored -- so no
+        // branch holds a load back.
+        const int ml = tid & 63, mw = tid >> 6;
+        const int kx = (p + 63) / 64, kt = kx + (it.q + 63) / 64;
+        auto tile_load = [&](double2 (&xa)[BC / 4], int k) {
+            const bool isv = k >= kx;
+            const double2* base = isv ? it.v : it.x;
+            const int rows = isv ? it.q : p;
+            const int r = min((isv ? k - kx : k) * 64 + 16 * mw + (ml & 15), rows - 1);
+#pragma unroll
+            for (int st = 0; st < BC / 4; ++st) {
+                const int c = sCol[4 * st + (ml >> 4)];
+                xa[st] = gload(base, (size_t)r + (size_t)rows * max(c, 0));
+            }
+        };
+        auto tile_update = [&](const double2 (&xa)[BC / 4], int k) {
+            f64x4 accP[2], accQ[2], accS[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int st = 0; st < BC / 4; ++st) {
+                double2 qb[2];
+#pragma unroll
+                for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
+                const double xs = xa[st].x + xa[st].y;
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const double qs = qb[b].x + qb[b].y;
+                    accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].x, qb[b].x, accP[b], 0, 0, 0);
+                    accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].y, qb[b].y, accQ[b], 0, 0, 0);
+                    accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qs, accS[b], 0, 0, 0);
+                }
+            }
+            const bool isv = k >= kx;
+            double2* base = isv ? it.v : it.x;
+            const int rows = isv ? it.q : p;
+            const int r0 = (isv ? k - kx : k) * 64 + 16 * mw + (ml >> 4);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int gc = sCol[16 * b + (ml & 15)];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int grow = r0 + 4 * q;
+                    if (gc >= 0 && grow < rows)
+                        base[grow + (size_t)rows * gc] =
+                            make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
+                }
+            }
+        };
+        double2 xa0[BC / 4], xa1[BC / 4];
+        tile_load(xa0, 0);
+#pragma unroll 1
+        for (int k = 0; k < kt; k += 2) {
+            if (k + 1 < kt) tile_load(xa1, k + 1);
+            tile_update(xa0, k);
+            if (k + 1 >= kt) break;
+            if (k + 2 < kt) tile_load(xa0, k + 2);
+            tile_update(xa1, k + 1);
+        }
+        return;
+    }
 #pragma unroll 1
     for (int mat = 0; mat < 2; ++mat) {
         double2* base = mat ? it.v : it.x;
         const int rows = mat ? it.q : p;
-        if constexpr (BC != 32) ufetch(base, rows, 0, 0);
+        ufetch(base, rows, 0, 0);
 #pragma unroll 1
         for (int row0 = 0; row0 < rows; row0 += RB) {
-          if constexpr (BC == 32) {
-            // On the matrix cores (v_mfma_f64_16x16x4): wave mw owns panel rows
-            // [32 mw, 32 mw + 32) as two 16-row tiles, times Q's two 16-column tiles.  A
-            // complex tile product is three real ones: P1 = Xr Qr, P2 = Xi Qi,
-            // P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.  Operand lanes:
-            // A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l is
-            // D[l / 16 + 4 q][l % 16].
-            const int ml = tid & 63, mw = tid >> 6;
-            f64x4 accP[2][2], accQ[2][2], accS[2][2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    accP[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-                    accQ[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-                    accS[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-                }
-            // A operands straight from HBM: the wave reads only its own rows (16
-            // consecutive rows of one column per 16 lanes), so the update needs no
-            // staging and no barrier; Q is read-only in LDS after the eigensolver.
-            // The loads are unconditional: a row past the end or an absent column reads
-            // an element in range instead -- absent columns meet zero rows of Q (their
-            // positions keep e_j), rows past the end are not stored -- so no branch
-            // holds a load back.
-            const int r0w = row0 + 32 * mw + (ml & 15);
-#pragma unroll
-            for (int st = 0; st < BC / 4; ++st) {
-                const int c = sCol[4 * st + (ml >> 4)];
-                double2 xa[2], qb[2];
-#pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const int r = r0w + 16 * a;
-                    xa[a] = gload(base, (size_t)min(r, rows - 1) + (size_t)rows * max(c, 0));
-                }
-#pragma unroll
-                for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
-                double xs[2], qs[2];
-#pragma unroll
-                for (int a = 0; a < 2; ++a) xs[a] = xa[a].x + xa[a].y;
-#pragma unroll
-                for (int b = 0; b < 2; ++b) qs[b] = qb[b].x + qb[b].y;
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        accP[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].x, qb[b].x, accP[a][b], 0, 0, 0);
-                        accQ[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a].y, qb[b].y, accQ[a][b], 0, 0, 0);
-                        accS[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[a], qs[b], accS[a][b], 0, 0, 0);
-                    }
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const int gc = sCol[16 * b + (ml & 15)];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int grow = row0 + 32 * mw + 16 * a + (ml >> 4) + 4 * q;
-                        if (gc >= 0 && grow < rows)
-                            base[grow + (size_t)rows * gc] =
-                                make_double2(accP[a][b][q] - accQ[a][b][q], accS[a][b][q] - accP[a][b][q] - accQ[a][b][q]);
-                    }
-                }
-          } else {
             double2 o[RA][JG];
 #pragma unroll
             for (int a = 0; a < RA; ++a)
@@ -629,7 +668,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if (gc >= 0 && grow < rows) base[grow + (size_t)rows * gc] = o[a][b];
                 }
             }
-          }
         }
     }
 }
