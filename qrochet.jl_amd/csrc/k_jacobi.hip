@@ -615,15 +615,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
             }
         };
-        double2 xa0[BC / 4], xa1[BC / 4];
+        double2 xa0[BC / 4], xa1[BC / 4], xa2[BC / 4];
         tile_load(xa0, 0);
+        if (1 < kt) tile_load(xa1, 1);
 #pragma unroll 1
-        for (int k = 0; k < kt; k += 2) {
-            if (k + 1 < kt) tile_load(xa1, k + 1);
+        for (int k = 0; k < kt; k += 3) {
+            if (k + 2 < kt) tile_load(xa2, k + 2);
             tile_update(xa0, k);
             if (k + 1 >= kt) break;
-            if (k + 2 < kt) tile_load(xa0, k + 2);
+            if (k + 3 < kt) tile_load(xa0, k + 3);
             tile_update(xa1, k + 1);
+            if (k + 2 >= kt) break;
+            if (k + 4 < kt) tile_load(xa1, k + 4);
+            tile_update(xa2, k + 2);
         }
         return;
     }
