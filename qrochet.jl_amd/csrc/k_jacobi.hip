@@ -969,53 +969,66 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
     __builtin_amdgcn_s_setprio(0);
 
     // ---------------- X_IJ <- X_IJ Q ,  V_IJ <- V_IJ Q  (matrix cores) ----------------
-    // 128-row panels: wave wv owns panel rows [16 wv, 16 wv + 16) times Q's four
-    // 16-column tiles; operands straight from HBM (each wave reads and writes only its
-    // own rows); three real products per complex tile: re = P1 - P2, im = P3 - P1 - P2.
-#pragma unroll 1
-    for (int mat = 0; mat < 2; ++mat) {
-        double2* base = mat ? it.v : it.x;
-        const int rows = mat ? it.q : p;
-#pragma unroll 1
-        for (int row0 = 0; row0 < rows; row0 += 128) {
-            f64x4 accP[4], accQ[4], accS[4];
+    // 128-row panels of X and then of V: wave wv owns panel rows [16 wv, 16 wv + 16)
+    // times Q's four 16-column tiles; operands straight from HBM (each wave reads and
+    // writes only its own rows), the next panel's sixteen loads issued before the
+    // current panel's products (unconditional, in range: see the width-16 update);
+    // three real products per complex tile: re = P1 - P2, im = P3 - P1 - P2.
+    const int kx = (p + 127) / 128, kt = kx + (it.q + 127) / 128;
+    auto panel_load = [&](double2 (&xall)[BC / 4], int k) {
+        const bool isv = k >= kx;
+        const double2* base = isv ? it.v : it.x;
+        const int rows = isv ? it.q : p;
+        const int rw = min((isv ? k - kx : k) * 128 + 16 * wv + (lane & 15), rows - 1);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
-            const int rw = row0 + 16 * wv + (lane & 15);
-            // all sixteen operand loads of the panel first, so they are in flight together
-            // (unconditional, in range: see the width-16 update)
-            double2 xall[BC / 4];
+        for (int st = 0; st < BC / 4; ++st) {
+            const int c = sCol[4 * st + (lane >> 4)];
+            xall[st] = gload(base, (size_t)rw + (size_t)rows * max(c, 0));
+        }
+    };
+    auto panel_update = [&](const double2 (&xall)[BC / 4], int k) {
+        f64x4 accP[4], accQ[4], accS[4];
 #pragma unroll
-            for (int st = 0; st < BC / 4; ++st) {
-                const int c = sCol[4 * st + (lane >> 4)];
-                xall[st] = gload(base, (size_t)min(rw, rows - 1) + (size_t)rows * max(c, 0));
-            }
+        for (int b = 0; b < 4; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int st = 0; st < BC / 4; ++st) {
-                // keep each step's Q reads in their step (the loads above stay in flight)
-                __builtin_amdgcn_sched_barrier(0);
-                const double2 xa = xall[st];
-                const double xs = xa.x + xa.y;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const double2 qv = sQ[4 * st + (lane >> 4)][16 * b + (lane & 15)];
-                    accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, qv.x, accP[b], 0, 0, 0);
-                    accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, qv.y, accQ[b], 0, 0, 0);
-                    accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qv.x + qv.y, accS[b], 0, 0, 0);
-                }
-            }
+        for (int st = 0; st < BC / 4; ++st) {
+            // keep each step's Q reads in their step (the loads stay in flight)
+            __builtin_amdgcn_sched_barrier(0);
+            const double2 xa = xall[st];
+            const double xs = xa.x + xa.y;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const int gc = sCol[16 * b + (lane & 15)];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int grow = row0 + 16 * wv + (lane >> 4) + 4 * q;
-                    if (gc >= 0 && grow < rows)
-                        base[grow + (size_t)rows * gc] =
-                            make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
-                }
+                const double2 qv = sQ[4 * st + (lane >> 4)][16 * b + (lane & 15)];
+                accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, qv.x, accP[b], 0, 0, 0);
+                accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, qv.y, accQ[b], 0, 0, 0);
+                accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qv.x + qv.y, accS[b], 0, 0, 0);
             }
         }
+        const bool isv = k >= kx;
+        double2* base = isv ? it.v : it.x;
+        const int rows = isv ? it.q : p;
+        const int r0 = (isv ? k - kx : k) * 128 + 16 * wv + (lane >> 4);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int gc = sCol[16 * b + (lane & 15)];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int grow = r0 + 4 * q;
+                if (gc >= 0 && grow < rows)
+                    base[grow + (size_t)rows * gc] =
+                        make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
+            }
+        }
+    };
+    double2 xa0[BC / 4], xa1[BC / 4];
+    panel_load(xa0, 0);
+#pragma unroll 1
+    for (int k = 0; k < kt; k += 2) {
+        if (k + 1 < kt) panel_load(xa1, k + 1);
+        panel_update(xa0, k);
+        if (k + 1 >= kt) break;
+        if (k + 2 < kt) panel_load(xa0, k + 2);
+        panel_update(xa1, k + 1);
     }
 }
 
