@@ -1231,6 +1231,10 @@ __global__ void k_conv_update(const JacItem* __restrict__ items, int count, int*
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         const JacItem it = items[i];
         const int cv = *it.conv;
+        // the window's rotation count and the tail state it ran under, for the host's
+        // accounting (read back with `remaining`)
+        remaining[4 + 2 * i] = *it.rot;
+        remaining[4 + 2 * i + 1] = *it.tail;
         if (cv == 0) {
             // an item whose own sweep ends with the window takes the round kernel's tail
             // decision now, so that the next window does not start with it

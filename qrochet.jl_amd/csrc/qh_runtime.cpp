@@ -427,8 +427,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     int rmax = 1;
     // per item: rotations in the window, converged, tail state, JacItem::own
     constexpr int CS = 3 + JAC_OWN_INTS;
-    int* counters = dalloc<int>(c, CS * items.size() + 4);
-    QH_HIP(hipMemsetAsync(counters, 0, (CS * items.size() + 4) * sizeof(int), c->stream));
+    // then `remaining` (4) and per item the window's rotations and tail state (2)
+    const size_t ncount = CS * items.size() + 4 + 2 * items.size();
+    int* counters = dalloc<int>(c, ncount);
+    QH_HIP(hipMemsetAsync(counters, 0, ncount * sizeof(int), c->stream));
     double* fro2 = fro2_out ? fro2_out : dalloc<double>(c, items.size());
     auto free_fro2 = [&] {
         if (!fro2_out) dfree(c, fro2);
@@ -520,9 +522,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         std::vector<int> hcnt(CS * items.size()), hflags(tail_total);
         std::vector<char> had_sets(items.size(), 0);
         auto publish = [&] {  // converged items, tail states, offsets, cleared rotation counts
-            QH_HIP(hipMemcpyAsync(counters, hcnt.data(), hcnt.size() * sizeof(int), hipMemcpyHostToDevice,
-                                  c->stream));
-            QH_HIP(hipStreamSynchronize(c->stream));  // hcnt is rewritten by the next pass
+            // through the descriptor arena (pinned staging that no later upload of this
+            // call overwrites), so the host need not wait for the copy
+            const int* staged = upload(c, hcnt);
+            QH_HIP(hipMemcpyAsync(counters, staged, hcnt.size() * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
         };
         // back to plain sweeps at the sweep it had not run; `again`: the first own sweep at
         // which it may enter the tail once more
@@ -709,22 +712,26 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
                              items[i].p, items[i].q, items[i].nb);
             std::fprintf(stderr, "\n");
         }
-        if (c->timing) {
-            readback(c, hrot.data(), counters, hrot.size() * sizeof(int));
-            auto& t = c->timers["jacobi_round"];
-            for (size_t i = 0; i < items.size(); ++i) {
-                if (converged[i] || hrot[CS * i + 2] == 1) continue;
-                const double checked = (double)(items[i].nb / 2) * rmax;
-                t.flops += checked * gram_f[i] + hrot[CS * i] * upd_f[i];
-                t.bytes += checked * gram_b[i] + hrot[CS * i] * upd_b[i];
-                if (hrot[CS * i] == 0) converged[i] = 1;
-            }
-        }
         launch_conv_update(di, (int)items.size(), remaining, round_base, rmax, sweep * rmax, c->jac_sort_sweeps,
                            tail_switch, c->stream);
         c->timers["jacobi_sweep"].launches += 1;
         int rem[4] = {0, 0, 0, 0};
-        readback(c, rem, remaining, sizeof(rem));
+        if (c->timing) {  // `remaining` and the window's per-item counts in one read-back
+            std::vector<int> hs(4 + 2 * items.size());
+            readback(c, hs.data(), remaining, hs.size() * sizeof(int));
+            std::copy(hs.begin(), hs.begin() + 4, rem);
+            auto& t = c->timers["jacobi_round"];
+            for (size_t i = 0; i < items.size(); ++i) {
+                const int wrot = hs[4 + 2 * i], wtail = hs[4 + 2 * i + 1];
+                if (converged[i] || wtail == 1) continue;
+                const double checked = (double)(items[i].nb / 2) * rmax;
+                t.flops += checked * gram_f[i] + wrot * upd_f[i];
+                t.bytes += checked * gram_b[i] + wrot * upd_b[i];
+                if (wrot == 0) converged[i] = 1;
+            }
+        } else {
+            readback(c, rem, remaining, sizeof(rem));
+        }
         if (rem[1] > 0) {
             std::vector<int> h(CS * items.size());
             readback(c, h.data(), counters, h.size() * sizeof(int));

@@ -91,12 +91,19 @@ class Dense:
     def canonical(self) -> np.ndarray:
         """Column-major flat buffer (out_1..out_k, in_1..in_k), lanes ascending."""
         lanes = self.lanes()
-        try:
-            order = [self.sites.index(Site(i, False)) for i in lanes] + [self.sites.index(Site(i, True)) for i in lanes]
-        except ValueError:
-            raise ArgumentError("Gate inputs and outputs must be the same") from None
-        g = np.transpose(np.asarray(self.array, dtype=np.complex128), order)
-        return np.asfortranarray(g).ravel(order="F").copy()
+        m = len(lanes)
+        g = np.asarray(self.array, dtype=np.complex128)
+        if len(self.sites) == 2 * m and all(s.id == lanes[k % m] and s.dual == (k >= m)
+                                            for k, s in enumerate(self.sites)):
+            flat = np.asfortranarray(g).ravel(order="F")  # already in canonical order
+        else:
+            try:
+                order = ([self.sites.index(Site(i, False)) for i in lanes] +
+                         [self.sites.index(Site(i, True)) for i in lanes])
+            except ValueError:
+                raise ArgumentError("Gate inputs and outputs must be the same") from None
+            flat = np.asfortranarray(np.transpose(g, order)).ravel(order="F")
+        return flat.copy() if np.may_share_memory(flat, self.array) else flat
 
     def copy(self):
         return Dense(self.socket, self.array.copy(), list(self.sites))
@@ -472,7 +479,7 @@ def _lanes_checked(psi: Chain, gate: Dense):
     outs = {s.id for s in gate.outputs}
     if ins != outs:
         raise ArgumentError(f"Gate inputs ({sorted(ins)}) and outputs ({sorted(outs)}) must be the same")
-    if not ins <= set(range(1, psi.n + 1)):
+    if not all(1 <= i <= psi.n for i in ins):
         raise ArgumentError(f"Gate inputs ({sorted(ins)}) must be a subset of the TN sites")
     lanes = sorted(ins)
     if len(lanes) == 2:
