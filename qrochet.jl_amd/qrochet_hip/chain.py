@@ -10,6 +10,7 @@ MPS tensors stay resident in device memory between calls.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import weakref
 from dataclasses import dataclass
 
@@ -109,6 +110,18 @@ class Dense:
         return Dense(self.socket, self.array.copy(), list(self.sites))
 
 
+_SITES: dict = {}
+
+
+def _site_of(i, dual):
+    """Site(i, dual), shared (sites are immutable)."""
+    key = (int(i), bool(dual))
+    s = _SITES.get(key)
+    if s is None:
+        s = _SITES[key] = Site(*key)
+    return s
+
+
 def gate(array_or_matrix, *lanes) -> Dense:
     """Operator on ``lanes``: a (d,…,d) array in (outs…, ins…) order, or a
     (d^k × d^k) matrix whose row/column index has the first lane fastest
@@ -116,9 +129,9 @@ def gate(array_or_matrix, *lanes) -> Dense:
     a = np.asarray(array_or_matrix, dtype=np.complex128)
     k = len(lanes)
     if a.ndim == 2 and k == 2:
-        d = int(round(np.sqrt(a.shape[0])))
+        d = int(round(math.sqrt(a.shape[0])))
         a = a.reshape((d, d, d, d), order="F")
-    sites = [Site(i) for i in lanes] + [Site(i, True) for i in lanes]
+    sites = [_site_of(i, False) for i in lanes] + [_site_of(i, True) for i in lanes]
     return Dense(Operator(), a, sites)
 
 
