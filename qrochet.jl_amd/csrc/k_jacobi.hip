@@ -562,9 +562,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // D[l / 16 + 4 q][l % 16].
         // A operands straight from HBM: the wave reads and writes only its own rows (16
         // consecutive rows of one column per 16 lanes), so the update needs no staging
-        // and no barrier; Q is read-only in LDS after the eigensolver.  The next tile's
-        // loads are issued before the current tile's products, so they arrive while the
-        // matrix cores work.  The loads are unconditional: a row past the end or an
+        // and no barrier; Q is read-only in LDS after the eigensolver.  Three tile buffers:
+        // the loads of the next two tiles are in flight while the matrix cores work on the
+        // current one.  The loads are unconditional: a row past the end or an
         // absent column reads an element in range instead -- absent columns meet zero
         // rows of Q (their positions keep e_j), rows past the end are not stored -- so no
         // branch holds a load back.
