@@ -105,7 +105,8 @@ int qh_ctx_set_pivoting(qh_context* ctx, int enable);
  * the batched SVD of `batch` synthetic q × q matrices (flat and decaying spectra;
  * fewer above q = 512) under the context's current schedule and every candidate --
  * Jacobi block width 8 or 16 and 1..4 QR preconditioning passes -- and keeps the
- * fastest per class for this context: a matrix then gets the passes of the smallest class >= its column count,
+ * fastest per class for this context if it beats the current schedule by at least 3 %
+ * (else the current schedule stays): a matrix then gets the passes of the smallest class >= its column count,
  * and a batch the block width of its largest member's class.  report (may be NULL):
  * 5 doubles per class (q, block, passes, best ms, ms of the previous setting), at most
  * 8 classes; nclasses (may be NULL) receives their number.  qh_ctx_set_jacobi and

@@ -121,6 +121,91 @@ __device__ __forceinline__ void rot_rows(double2& ya, double2& yb, const JRot& j
     ya = na;
 }
 
+// X_IJ <- X_IJ Q and V_IJ <- V_IJ Q for the 16-row tiles k0 <= k < k1 of X (k < kx =
+// ceil(p / 64)) followed by V (the pair's columns in sCol, Q in sQ, both in LDS).
+// On the matrix cores (v_mfma_f64_16x16x4), in 16-row tiles: wave mw takes rows
+// [64 k + 16 mw, 64 k + 16 mw + 16) of X (k < kx) and then of V, times Q's two
+// 16-column tiles.  A complex tile product is three real ones: P1 = Xr Qr,
+// P2 = Xi Qi, P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.  Operand
+// lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l is
+// D[l / 16 + 4 q][l % 16].
+// A operands straight from HBM: the wave reads and writes only its own rows (16
+// consecutive rows of one column per 16 lanes), so the update needs no staging
+// and no barrier; Q is read-only in LDS after the eigensolver.  Three tile buffers:
+// the loads of the next two tiles are in flight while the matrix cores work on the
+// current one.  The loads are unconditional: a row past the end or an
+// absent column reads an element in range instead -- absent columns meet zero
+// rows of Q (their positions keep e_j), rows past the end are not stored -- so no
+// branch holds a load back.
+template <int BC>
+__device__ __forceinline__ void jac_tiles_update(const JacItem& it, const int* sCol, const double2 (*sQ)[BC + 1],
+                                                 int k0, int k1) {
+    const int tid = threadIdx.x;
+    const int p = it.p;
+    const int kx = (p + 63) / 64;
+    const int ml = tid & 63, mw = tid >> 6;
+    auto tile_load = [&](double2 (&xa)[BC / 4], int k) {
+        const bool isv = k >= kx;
+        const double2* base = isv ? it.v : it.x;
+        const int rows = isv ? it.q : p;
+        const int r = min((isv ? k - kx : k) * 64 + 16 * mw + (ml & 15), rows - 1);
+#pragma unroll
+        for (int st = 0; st < BC / 4; ++st) {
+            const int c = sCol[4 * st + (ml >> 4)];
+            xa[st] = gload(base, (size_t)r + (size_t)rows * max(c, 0));
+        }
+    };
+    auto tile_update = [&](const double2 (&xa)[BC / 4], int k) {
+        f64x4 accP[2], accQ[2], accS[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int st = 0; st < BC / 4; ++st) {
+            double2 qb[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
+            const double xs = xa[st].x + xa[st].y;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const double qs = qb[b].x + qb[b].y;
+                accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].x, qb[b].x, accP[b], 0, 0, 0);
+                accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].y, qb[b].y, accQ[b], 0, 0, 0);
+                accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qs, accS[b], 0, 0, 0);
+            }
+        }
+        const bool isv = k >= kx;
+        double2* base = isv ? it.v : it.x;
+        const int rows = isv ? it.q : p;
+        const int r0 = (isv ? k - kx : k) * 64 + 16 * mw + (ml >> 4);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int gc = sCol[16 * b + (ml & 15)];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int grow = r0 + 4 * q;
+                if (gc >= 0 && grow < rows)
+                    base[grow + (size_t)rows * gc] =
+                        make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
+            }
+        }
+    };
+    if (k0 >= k1) return;
+    double2 xa0[BC / 4], xa1[BC / 4], xa2[BC / 4];
+    tile_load(xa0, k0);
+    if (k0 + 1 < k1) tile_load(xa1, k0 + 1);
+#pragma unroll 1
+    for (int k = k0; k < k1; k += 3) {
+        if (k + 2 < k1) tile_load(xa2, k + 2);
+        tile_update(xa0, k);
+        if (k + 1 >= k1) break;
+        if (k + 3 < k1) tile_load(xa0, k + 3);
+        tile_update(xa1, k + 1);
+        if (k + 2 >= k1) break;
+        if (k + 4 < k1) tile_load(xa1, k + 4);
+        tile_update(xa2, k + 2);
+    }
+}
+
 // CHECK: the test of every (round, pair) of a sweep in one launch, without rotating (its
 // own kernel, so that profiles and the round timings keep the rounds apart)
 template <int BC, bool CHECK>
@@ -130,7 +215,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                       const int* __restrict__ round_base, int sort_sweeps,
                                                       int debug, int max_inner, int mode,
                                                       int* __restrict__ flags, const int* __restrict__ flag0,
-                                                      int tail_switch) {
+                                                      int tail_switch, cplx* __restrict__ vq,
+                                                      int* __restrict__ vhdr, const int* __restrict__ vmap,
+                                                      int nt_pairs, int nv) {
     constexpr int B = BC / 2;
     constexpr int LP = BC + 1;
     constexpr int CH = 32;
@@ -155,6 +242,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ int sGo;
 
     const int tid = threadIdx.x;
+    // the global round of this launch (for JAC_ROUND and JAC_VONLY)
+    const int g = round_base ? *round_base + round_arg : round_arg;
+    if constexpr (BC == 32 && !CHECK) {
+        // V workgroups (with vq: workgroups nt_pairs.. of a round launch, every workgroup
+        // of a JAC_VONLY launch): workgroup v applies V_IJ <- V_IJ Q for the Q that pair
+        // vmap[nt_pairs + v] stored in the previous round (slot parity (g - 1) & 1) and
+        // marks the slot consumed.  The round's own workgroups touch X and the other
+        // parity's slots only.
+        if (vq != nullptr && (mode == JAC_VONLY || (int)blockIdx.x >= nt_pairs)) {
+            const int v = (mode == JAC_VONLY) ? (int)blockIdx.x : (int)blockIdx.x - nt_pairs;
+            const int t = vmap[nt_pairs + v];
+            const int slot = ((g - 1) & 1) * nv + v;
+            if (vhdr[4 * slot] == 0) return;
+            const JacItem itv = items[tiles[t].x];
+            if (tid < BC) {
+                const int c = (tid < B) ? vhdr[4 * slot + 1] * B + tid : vhdr[4 * slot + 2] * B + (tid - B);
+                sCol[tid] = (c < itv.q) ? c : -1;
+            }
+            for (int e = tid; e < BC * BC; e += 256) sQ[e / BC][e % BC] = vq[(size_t)slot * BC * BC + e];
+            __syncthreads();  // (every thread has read the flag)
+            const int kx = (itv.p + 63) / 64;
+            jac_tiles_update<BC>(itv, sCol, sQ, kx, kx + (itv.q + 63) / 64);
+            if (tid == 0) vhdr[4 * slot] = 0;
+            return;
+        }
+    }
     const int2 tt = tiles[blockIdx.x];
     const JacItem it = items[tt.x];
     if (*((volatile int*)it.conv)) return;
@@ -181,7 +294,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         // device memory plus the argument (so that a captured sweep replays with
         // advancing rounds); eigenpair sorting during this matrix's own first sweeps
         // (independent of the batch)
-        round = (round_base ? *round_base + round_arg : round_arg) - it.own[JAC_ROUND_OFF];
+        round = g - it.own[JAC_ROUND_OFF];
         own = round / R;
         sort = round < sort_sweeps * R;
         rr_pair(round % R, tt.y, it.nb, bi, bj);
@@ -554,81 +667,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     };
     if constexpr (BC == 32) {
-        // On the matrix cores (v_mfma_f64_16x16x4), in 16-row tiles: wave mw takes rows
-        // [64 k + 16 mw, 64 k + 16 mw + 16) of X (k < kx) and then of V, times Q's two
-        // 16-column tiles.  A complex tile product is three real ones: P1 = Xr Qr,
-        // P2 = Xi Qi, P3 = (Xr + Xi)(Qr + Qi); re = P1 - P2, im = P3 - P1 - P2.  Operand
-        // lanes: A[l % 16][l / 16], B[l / 16][l % 16]; result register q of lane l is
-        // D[l / 16 + 4 q][l % 16].
-        // A operands straight from HBM: the wave reads and writes only its own rows (16
-        // consecutive rows of one column per 16 lanes), so the update needs no staging
-        // and no barrier; Q is read-only in LDS after the eigensolver.  Three tile buffers:
-        // the loads of the next two tiles are in flight while the matrix cores work on the
-        // current one.  The loads are unconditional: a row past the end or an
-        // absent column reads an element in range instead -- absent columns meet zero
-        // rows of Q (their positions keep e_j), rows past the end are not stored -- so no
-        // branch holds a load back.
-        const int ml = tid & 63, mw = tid >> 6;
+        // X (and V unless V_IJ Q is left to the next launch's V workgroups) on the matrix
+        // cores, in 16-row tiles
         const int kx = (p + 63) / 64, kt = kx + (it.q + 63) / 64;
-        auto tile_load = [&](double2 (&xa)[BC / 4], int k) {
-            const bool isv = k >= kx;
-            const double2* base = isv ? it.v : it.x;
-            const int rows = isv ? it.q : p;
-            const int r = min((isv ? k - kx : k) * 64 + 16 * mw + (ml & 15), rows - 1);
-#pragma unroll
-            for (int st = 0; st < BC / 4; ++st) {
-                const int c = sCol[4 * st + (ml >> 4)];
-                xa[st] = gload(base, (size_t)r + (size_t)rows * max(c, 0));
+        const int v = (vq != nullptr && mode == JAC_ROUND) ? vmap[blockIdx.x] : -1;
+        const bool defer = v >= 0;
+        if (defer) {  // Q and its block pair for the V workgroup of the next launch
+            const int slot = (g & 1) * nv + v;
+            for (int e = tid; e < BC * BC; e += 256) vq[(size_t)slot * BC * BC + e] = sQ[e / BC][e % BC];
+            if (tid == 0) {
+                vhdr[4 * slot + 1] = bi;
+                vhdr[4 * slot + 2] = bj;
+                vhdr[4 * slot] = 1;
             }
-        };
-        auto tile_update = [&](const double2 (&xa)[BC / 4], int k) {
-            f64x4 accP[2], accQ[2], accS[2];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) accP[b] = accQ[b] = accS[b] = f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int st = 0; st < BC / 4; ++st) {
-                double2 qb[2];
-#pragma unroll
-                for (int b = 0; b < 2; ++b) qb[b] = sQ[4 * st + (ml >> 4)][16 * b + (ml & 15)];
-                const double xs = xa[st].x + xa[st].y;
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const double qs = qb[b].x + qb[b].y;
-                    accP[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].x, qb[b].x, accP[b], 0, 0, 0);
-                    accQ[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[st].y, qb[b].y, accQ[b], 0, 0, 0);
-                    accS[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, qs, accS[b], 0, 0, 0);
-                }
-            }
-            const bool isv = k >= kx;
-            double2* base = isv ? it.v : it.x;
-            const int rows = isv ? it.q : p;
-            const int r0 = (isv ? k - kx : k) * 64 + 16 * mw + (ml >> 4);
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int gc = sCol[16 * b + (ml & 15)];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int grow = r0 + 4 * q;
-                    if (gc >= 0 && grow < rows)
-                        base[grow + (size_t)rows * gc] =
-                            make_double2(accP[b][q] - accQ[b][q], accS[b][q] - accP[b][q] - accQ[b][q]);
-                }
-            }
-        };
-        double2 xa0[BC / 4], xa1[BC / 4], xa2[BC / 4];
-        tile_load(xa0, 0);
-        if (1 < kt) tile_load(xa1, 1);
-#pragma unroll 1
-        for (int k = 0; k < kt; k += 3) {
-            if (k + 2 < kt) tile_load(xa2, k + 2);
-            tile_update(xa0, k);
-            if (k + 1 >= kt) break;
-            if (k + 3 < kt) tile_load(xa0, k + 3);
-            tile_update(xa1, k + 1);
-            if (k + 2 >= kt) break;
-            if (k + 4 < kt) tile_load(xa1, k + 4);
-            tile_update(xa2, k + 2);
         }
+        jac_tiles_update<BC>(it, sCol, sQ, 0, defer ? kx : kt);
         return;
     }
 #pragma unroll 1
@@ -1034,29 +1087,35 @@ __global__ __launch_bounds__(512) void k_jacobi_round64(const JacItem* __restric
 
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
                          const int* round_base, int block, int sort_sweeps, int debug, int max_inner, int tail_switch,
-                         hipStream_t s) {
+                         cplx* vq, int* vhdr, const int* vmap, int nv, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
         k_jacobi_round<16, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug,
                                                          max_inner, JAC_ROUND, nullptr, nullptr,
-                                                         tail_switch);
+                                                         tail_switch, nullptr, nullptr, nullptr, ntiles, 0);
     else if (block == 32)
         k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, max_inner,
                                                 JAC_ROUND, tail_switch);
-    else
-        k_jacobi_round<32, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, round, round_base, sort_sweeps, debug,
-                                                         max_inner, JAC_ROUND, nullptr, nullptr,
-                                                         tail_switch);
+    else  // with vq: the pairs' workgroups, then the V workgroups
+        k_jacobi_round<32, false><<<vq ? ntiles + nv : ntiles, 256, 0, s>>>(
+            d_items, d_tiles, round, round_base, sort_sweeps, debug, max_inner, JAC_ROUND, nullptr, nullptr,
+            tail_switch, vq, vhdr, vmap, ntiles, nv);
+}
+void launch_jacobi_vonly(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, const int* round_base,
+                         cplx* vq, int* vhdr, const int* vmap, int nv, hipStream_t s) {
+    if (nv <= 0) return;
+    k_jacobi_round<32, false><<<nv, 256, 0, s>>>(d_items, d_tiles, round, round_base, 0, 0, 1, JAC_VONLY, nullptr,
+                                                 nullptr, 0, vq, vhdr, vmap, ntiles, nv);
 }
 void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles, int* d_flags, const int* d_flag0,
                          int block, hipStream_t s) {
     if (ntiles <= 0) return;
     if (block == 8)
         k_jacobi_round<16, true><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, 1, JAC_CHECK, d_flags, d_flag0,
-                                                        0);
+                                                        0, nullptr, nullptr, nullptr, ntiles, 0);
     else
         k_jacobi_round<32, true><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, 1, JAC_CHECK, d_flags, d_flag0,
-                                                        0);
+                                                        0, nullptr, nullptr, nullptr, ntiles, 0);
 }
 // Convergence test of whole items without rotating (block width 16 or 32): G = X^H X in
 // 64 × 64 tiles of the upper triangle of 16-column blocks, one workgroup per tile, on the
@@ -1158,10 +1217,10 @@ void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles
         k_jacobi_round64<<<ntiles, 512, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, max_inner, JAC_PAIRS, 0);
     else if (block == 8)
         k_jacobi_round<16, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, max_inner, JAC_PAIRS,
-                                                         nullptr, nullptr, 0);
+                                                         nullptr, nullptr, 0, nullptr, nullptr, nullptr, ntiles, 0);
     else
         k_jacobi_round<32, false><<<ntiles, 256, 0, s>>>(d_items, d_tiles, 0, nullptr, 0, 0, max_inner, JAC_PAIRS,
-                                                         nullptr, nullptr, 0);
+                                                         nullptr, nullptr, 0, nullptr, nullptr, nullptr, ntiles, 0);
 }
 
 // ||X||_F^2 of every item, deterministic: each workgroup reduces one chunk into

@@ -125,9 +125,20 @@ constexpr int FROB_CHUNK = 65536;
 // tail_switch > 0 (per mille): at the start of each of its own sweeps after
 // the sorting ones, an item whose previous sweep rotated at most that share of its block
 // pairs stops (tail = 1) and waits for the tail passes (JAC_CHECK / JAC_PAIRS launches)
+// vq / vhdr / vmap (block width 16 only; vq may be null): for the nv pairs of matrices
+// with at least JAC_DEFER_MIN_Q columns, V_IJ <- V_IJ Q is left to the V workgroups of the
+// next launch.  vmap[t] (t < ntiles) is pair t's V index v or -1, vmap[ntiles + v] the
+// pair of V index v.  Pair t stores Q (2·16 × 2·16) and its blocks in slot
+// (g & 1)·nv + v (g the global round; 2·nv slots of Q, 4 ints of header each: flag, bi, bj),
+// and the launch carries nv V workgroups that apply the previous round's slots.  A window
+// of rounds ends with launch_jacobi_vonly (round = the window's end), which applies the
+// last round's.  The headers start at zero.
+constexpr int JAC_DEFER_MIN_Q = 256;
 void launch_jacobi_round(const JacItem* d_items, const int2* d_tiles, int ntiles, int round,
                          const int* round_base, int block, int sort_sweeps, int debug, int max_inner, int tail_switch,
-                         hipStream_t s);
+                         cplx* vq, int* vhdr, const int* vmap, int nv, hipStream_t s);
+void launch_jacobi_vonly(const JacItem* d_items, const int2* d_tiles, int ntiles, int round, const int* round_base,
+                         cplx* vq, int* vhdr, const int* vmap, int nv, hipStream_t s);
 // Pair k of round `round` of the circle-method tournament over nidx (even) blocks.
 __host__ __device__ inline void rr_pair(int round, int k, int nidx, int& a, int& b) {
     const int m = nidx - 1;
@@ -143,7 +154,7 @@ __host__ __device__ inline void rr_pair(int round, int k, int nidx, int& a, int&
 // pair) of a sweep in one launch, without rotating (tile: item, round · nb/2 + pair;
 // flags[flag0[item] + tile.y] = 1 where the pair needs a rotation); explicit block
 // pairs (tile: item, bi | bj << 16), which must be disjoint within an item.
-enum JacMode { JAC_ROUND = 0, JAC_CHECK = 1, JAC_PAIRS = 2 };
+enum JacMode { JAC_ROUND = 0, JAC_CHECK = 1, JAC_PAIRS = 2, JAC_VONLY = 3 };
 void launch_jacobi_check(const JacItem* d_items, const int2* d_tiles, int ntiles, int* d_flags, const int* d_flag0,
                          int block, hipStream_t s);
 void launch_jacobi_pairs(const JacItem* d_items, const int2* d_tiles, int ntiles, int block, int max_inner,

@@ -667,6 +667,38 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     // runs the next sweep's rounds (the same indices as direct launches).
     const bool use_graph = c->use_graphs && !kdbg;
     int* round_base = use_graph ? remaining + 2 : nullptr;
+    // Block width 16: for matrices of at least JAC_DEFER_MIN_Q columns each round's
+    // V_IJ <- V_IJ Q runs in the next launch, beside the next round's pairs (two parities of
+    // Q slots, 16 KB each, and 4-int headers that start at zero); the window ends with a
+    // launch of V workgroups alone.  Smaller matrices update V in their pairs' workgroups.
+    // Only where all pairs fit on the device at once (three workgroups per CU) and those
+    // matrices hold at least half of them: the V workgroups follow all pairs in a launch,
+    // and behind more pairs than the device holds they start last and lengthen every
+    // launch (measured: the uneven 512-MPS workload, thousands of pairs, -5 %).
+    cplx* vq = nullptr;
+    int* vhdr = nullptr;
+    int* dvmap = nullptr;
+    int nv = 0;
+    if (block == 16 && c->jac_defer_v && tiles.size() <= 3 * (size_t)c->num_cus) {
+        std::vector<int> vmap(tiles.size(), -1), vlist;
+        for (size_t t = 0; t < tiles.size(); ++t)
+            if (items[tiles[t].x].q >= JAC_DEFER_MIN_Q) {
+                vmap[t] = (int)vlist.size();
+                vlist.push_back((int)t);
+            }
+        nv = 2 * vlist.size() >= tiles.size() ? (int)vlist.size() : 0;
+        if (nv > 0) {
+            vmap.insert(vmap.end(), vlist.begin(), vlist.end());
+            dvmap = upload(c, vmap);
+            vq = dalloc<cplx>(c, 2 * (size_t)nv * bc * bc);
+            vhdr = dalloc<int>(c, 8 * (size_t)nv);
+            QH_HIP(hipMemsetAsync(vhdr, 0, 8 * (size_t)nv * sizeof(int), c->stream));
+        }
+    }
+    auto free_vq = [&] {
+        if (vq) dfree(c, vq);
+        if (vhdr) dfree(c, vhdr);
+    };
     struct SweepGraph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -681,13 +713,16 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
             // one event pair per window of back-to-back rounds (per-launch events would
             // cost more host time than the short rounds themselves)
             Timed tm(c, "jacobi_round");
-            if (c->timing) c->timers["jacobi_round"].launches += rmax - 1;  // Timed counts one
+            if (c->timing) c->timers["jacobi_round"].launches += rmax - 1 + (vq ? 1 : 0);  // Timed counts one
             if (use_graph) {
                 if (!sg.exec) {
                     QH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                     for (int r = 0; r < rmax; ++r)
                         launch_jacobi_round(di, dt, (int)tiles.size(), r, round_base, block, c->jac_sort_sweeps, 0,
-                                            c->jac_max_inner, tail_switch, c->stream);
+                                            c->jac_max_inner, tail_switch, vq, vhdr, dvmap, nv, c->stream);
+                    if (vq)
+                        launch_jacobi_vonly(di, dt, (int)tiles.size(), rmax, round_base, vq, vhdr, dvmap, nv,
+                                            c->stream);
                     const hipError_t e1 = hipGetLastError();
                     const hipError_t e2 = hipStreamEndCapture(c->stream, &sg.graph);
                     QH_HIP(e1);
@@ -699,8 +734,11 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
                 for (int r = 0; r < rmax; ++r) {
                     const int g = sweep * rmax + r;
                     launch_jacobi_round(di, dt, (int)tiles.size(), g, nullptr, block, c->jac_sort_sweeps, kdbg,
-                                        c->jac_max_inner, tail_switch, c->stream);
+                                        c->jac_max_inner, tail_switch, vq, vhdr, dvmap, nv, c->stream);
                 }
+                if (vq)
+                    launch_jacobi_vonly(di, dt, (int)tiles.size(), (sweep + 1) * rmax, nullptr, vq, vhdr, dvmap, nv,
+                                        c->stream);
             }
         }
         static const bool dbg = std::getenv("QH_DEBUG_JACOBI") != nullptr;
@@ -739,9 +777,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
             for (size_t i = 0; i < items.size(); ++i)
                 if (h[CS * i + 1] < 0) msg += " item " + std::to_string(i);
             dfree(c, counters);
-        if (tail_flags) dfree(c, tail_flags);
+            if (tail_flags) dfree(c, tail_flags);
             free_fro2();
             dfree(c, frob_part);
+            free_vq();
             throw Error(QH_ERR_NUMERIC, msg);
         }
         if (rem[3] > 0) {  // items waiting for the tail passes
@@ -789,12 +828,14 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
         if (tail_flags) dfree(c, tail_flags);
         free_fro2();
         dfree(c, frob_part);
+        free_vq();
         throw Error(QH_ERR_NOT_CONVERGED, msg);
     }
     dfree(c, counters);
     if (tail_flags) dfree(c, tail_flags);
     free_fro2();
     dfree(c, frob_part);
+    free_vq();
 }
 
 // ---------------------------------------------------------------------------

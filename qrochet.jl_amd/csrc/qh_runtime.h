@@ -36,6 +36,7 @@ struct TimerClass {
 
 struct qh_context {
     int device = 0;
+    int num_cus = 256;  // compute units of the device (queried at creation)
     hipStream_t stream = nullptr;
     std::string err;
     // descriptor arena: chunks of pinned host staging + device mirror.  Within one
@@ -76,6 +77,7 @@ struct qh_context {
     int jac_max_inner = 1;    // sweep cap of the block-pair eigensolver (matrices with > 1 block pair)
     bool use_graphs = true;   // a Jacobi sweep's rounds captured once and replayed as a HIP graph
     bool jac_tail = true;     // the last sweeps as whole-sweep checks plus launches of the flagged pairs
+    bool jac_defer_v = true;  // block width 16: V_IJ Q of large matrices in the next launch's V workgroups
     bool pivot = true;        // first QR pass on the columns ordered by decreasing norm
     // set by qh_ctx_autotune: QR passes per size class (columns up to the key) and the
     // Jacobi block width per largest class of a batch; empty = the settings above
