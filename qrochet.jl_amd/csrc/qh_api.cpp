@@ -1107,6 +1107,8 @@ int qh_ctx_create(int device, qh_context** out) {
         if (const char* e = std::getenv("QH_PIVOT")) c->pivot = std::atoi(e) != 0;
         if (const char* e = std::getenv("QH_JACOBI_TAIL")) c->jac_tail = std::atoi(e) != 0;
         if (const char* e = std::getenv("QH_JACOBI_DEFER_V")) c->jac_defer_v = std::atoi(e) != 0;
+        if (const char* e = std::getenv("QH_JACOBI_TAIL_SWITCH"))
+            c->jac_tail_switch = std::min(1000, std::max(1, std::atoi(e)));
         if (const char* e = std::getenv("QH_JACOBI_BLOCK")) {
             const int b = std::atoi(e);
             if (b == 8 || b == 16 || b == 32) c->jac_block = b;

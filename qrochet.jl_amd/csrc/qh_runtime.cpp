@@ -487,8 +487,9 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     std::vector<int> hrot(CS * items.size());
     bool done = false;
     // The tail.  Once one of its own sweeps (after the sorting ones) rotated at most
-    // TAIL_SWITCH of its block pairs -- near the quadratic phase, where the next sweep
-    // rotates only a few -- an item leaves the rounds (the round kernel decides, per item,
+    // c->jac_tail_switch per mille (700 by default) of its block pairs -- near the quadratic
+    // phase, where the next sweep rotates only a few -- an item leaves the rounds (the
+    // round kernel decides, per item,
     // so the decision does not depend on the batch), and after the window its remaining
     // sweeps become passes of: one launch testing every (round, pair) of a sweep without
     // rotating; its flagged pairs grouped on the host into sets of disjoint pairs
@@ -499,10 +500,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     // left them (its round offset makes its next round the first of the sweep it had not
     // run; it may enter the tail again one sweep later), and so does an item still
     // flagged after TAIL_PASSES passes (for good).
-    constexpr int TAIL_SWITCH = 700, TAIL_MAX = 250;  // per mille of an item's block pairs
+    constexpr int TAIL_MAX = 250;  // per mille of an item's block pairs
     constexpr int TAIL_PASSES = 8;
     static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
-    const int tail_switch = (c->jac_tail && !kdbg) ? TAIL_SWITCH : 0;
+    const int tail_switch = (c->jac_tail && !kdbg) ? c->jac_tail_switch : 0;
     int* tail_flags = nullptr;
     std::vector<int> tail_first(items.size());
     size_t tail_total = 0;

@@ -78,6 +78,7 @@ struct qh_context {
     bool use_graphs = true;   // a Jacobi sweep's rounds captured once and replayed as a HIP graph
     bool jac_tail = true;     // the last sweeps as whole-sweep checks plus launches of the flagged pairs
     bool jac_defer_v = true;  // block width 16: V_IJ Q of large matrices in the next launch's V workgroups
+    int jac_tail_switch = 700;  // per mille of an item's block pairs: at most that many rotated -> tail
     bool pivot = true;        // first QR pass on the columns ordered by decreasing norm
     // set by qh_ctx_autotune: QR passes per size class (columns up to the key) and the
     // Jacobi block width per largest class of a batch; empty = the settings above
