@@ -489,10 +489,10 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     // The tail.  Once one of its own sweeps (after the sorting ones) rotated at most
     // c->jac_tail_switch per mille (700 by default) of its block pairs -- near the quadratic
     // phase, where the next sweep rotates only a few -- an item leaves the rounds (the
-    // round kernel decides, per item,
-    // so the decision does not depend on the batch), and after the window its remaining
-    // sweeps become passes of: one launch testing every (round, pair) of a sweep without
-    // rotating; its flagged pairs grouped on the host into sets of disjoint pairs
+    // round kernel decides, per item, so the decision does not depend on the batch), and
+    // after the window its remaining sweeps become passes of: one launch testing every
+    // (round, pair) of a sweep without rotating; its flagged pairs grouped on the host
+    // into sets of disjoint pairs
     // (greedily, in round order); one launch per set, where each pair is tested again on
     // its fresh Gram before it rotates.  A pass whose test flags nothing is a full sweep
     // without a rotation: the item has converged, as in the plain loop.  An item whose
