@@ -488,18 +488,17 @@ void run_jacobi(qh_context* c, std::vector<JacItem>& items, double* fro2_out, in
     bool done = false;
     // The tail.  Once one of its own sweeps (after the sorting ones) rotated at most
     // c->jac_tail_switch per mille (700 by default) of its block pairs -- near the quadratic
-    // phase, where the next sweep rotates only a few -- an item leaves the rounds (the
-    // round kernel decides, per item, so the decision does not depend on the batch), and
-    // after the window its remaining sweeps become passes of: one launch testing every
-    // (round, pair) of a sweep without rotating; its flagged pairs grouped on the host
-    // into sets of disjoint pairs
-    // (greedily, in round order); one launch per set, where each pair is tested again on
+    // phase, where the next sweep rotates only a few -- an item leaves the rounds (the round
+    // kernel decides, per item, so the decision does not depend on the batch), and after the
+    // window its remaining sweeps become passes of: one launch testing every (round, pair) of
+    // a sweep without rotating; its flagged pairs grouped on the host into sets of disjoint
+    // pairs (greedily, in round order); one launch per set, where each pair is tested again on
     // its fresh Gram before it rotates.  A pass whose test flags nothing is a full sweep
-    // without a rotation: the item has converged, as in the plain loop.  An item whose
-    // first test flags more than TAIL_MAX of its pairs returns to plain sweeps where it
-    // left them (its round offset makes its next round the first of the sweep it had not
-    // run; it may enter the tail again one sweep later), and so does an item still
-    // flagged after TAIL_PASSES passes (for good).
+    // without a rotation: the item has converged, as in the plain loop.  An item whose first
+    // test flags more than TAIL_MAX of its pairs returns to plain sweeps where it left them
+    // (its round offset makes its next round the first of the sweep it had not run; it may
+    // enter the tail again one sweep later), and so does an item still flagged after
+    // TAIL_PASSES passes (for good).
     constexpr int TAIL_MAX = 250;  // per mille of an item's block pairs
     constexpr int TAIL_PASSES = 8;
     static const int kdbg = std::getenv("QH_DEBUG_JACOBI_KERNEL") ? 1 : 0;
